@@ -1,0 +1,101 @@
+/* Host graph library: the GGPUCSR1 .bin format, generators, transforms and
+ * the CPU BFS / PageRank variants that gbench runs beside the GPU.
+ *
+ * Replaces, for C callers:
+ *   graph::CSR, load/save/validate/equal      src/graph/csr.hpp:29-70
+ *   transpose/symmetrize/permute              src/graph/transform.hpp:11-19
+ *   gen::path/star/.../rmat/suite             src/graph/generators.hpp:12-24
+ *   reorder::compute(..., "random", seed)     src/graph/reorder.cpp:20-25, :94-100
+ *   cpu::bfsSerial/bfsOmp/bfsDO               src/cpu/bfs_cpu.hpp:29-39
+ *   cpu::pagerankSerial/pagerankOmp           src/cpu/pagerank_cpu.hpp:32-33
+ *   gbench pickSources/largestComponent       src/bench/main.cpp:130-169
+ * Graphs are opaque host objects; every function returns a GGPU_* status and
+ * ggpu_host_last_error() describes the last failure on the calling thread. */
+#ifndef GGPU_GRAPH_H
+#define GGPU_GRAPH_H
+
+#include "ggpu_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ggpu_graph ggpu_graph;
+
+const char* ggpu_host_last_error(void);
+
+/* ---- construction / format (src/graph/csr.hpp:47-70, csr.cpp:30-163) ---- */
+int ggpu_graph_load(const char* path, ggpu_graph** out);
+int ggpu_graph_save(const ggpu_graph* g, const char* path);
+/* Copies the arrays. row_ptr has n+1 entries, col_idx m entries. */
+int ggpu_graph_from_arrays(uint64_t n, uint64_t m, const uint32_t* row_ptr, const uint32_t* col_idx,
+                           uint32_t flags, ggpu_graph** out);
+/* Edge list -> CSR; undirected edges are stored both ways; lists sorted,
+ * deduplicated, self-loops dropped (graph::fromEdgeList, csr.cpp:146-163). */
+int ggpu_graph_from_edges(uint64_t n, uint64_t num_edges, const uint32_t* src, const uint32_t* dst,
+                          int directed, ggpu_graph** out);
+void ggpu_graph_free(ggpu_graph* g);
+uint64_t ggpu_graph_n(const ggpu_graph* g);
+uint64_t ggpu_graph_m(const ggpu_graph* g);
+uint32_t ggpu_graph_flags(const ggpu_graph* g);
+const uint32_t* ggpu_graph_row_ptr(const ggpu_graph* g);
+const uint32_t* ggpu_graph_col_idx(const ggpu_graph* g);
+/* "" (returns GGPU_OK) if structurally valid, else a description in buf. */
+int ggpu_graph_validate(const ggpu_graph* g, char* buf, size_t buflen);
+int ggpu_graph_equal(const ggpu_graph* a, const ggpu_graph* b);
+
+/* ---- transforms (src/graph/transform.cpp:9-76) ---- */
+int ggpu_graph_transpose(const ggpu_graph* g, ggpu_graph** out);
+int ggpu_graph_symmetrize(const ggpu_graph* g, ggpu_graph** out);
+/* vertex v becomes perm[v] */
+int ggpu_graph_permute(const ggpu_graph* g, const uint32_t* perm, ggpu_graph** out);
+/* Fisher-Yates permutation with the SplitMix64 generator (reorder.cpp:20-25). */
+int ggpu_random_perm(uint64_t n, uint64_t seed, uint32_t* perm_out);
+/* Largest connected component of a symmetric graph, relabeled in increasing
+ * original-id order (row-major for grids); old_ids_out (optional, n entries)
+ * receives the original id of each new vertex. */
+int ggpu_graph_largest_component(const ggpu_graph* g, ggpu_graph** out, uint32_t* old_ids_out,
+                                 uint64_t* n_out);
+
+/* ---- generators (src/graph/generators.cpp:8-88, extended) ----
+ * name: path1000 star100k complete200 two_components empty1000 grid100x100
+ *       rmat12 rmat12_directed (the reference suite) and the workload graphs
+ *       rmat16_c1, rmat22_c2, grid_c3, rmat23_c5 (see DESIGN.md for their recipes). */
+int ggpu_gen_named(const char* name, ggpu_graph** out);
+int ggpu_gen_rmat(uint32_t scale, uint32_t edgefactor, double a, double b, double c, int directed,
+                  uint64_t seed, ggpu_graph** out);
+/* rows x cols 4-neighbour grid; each undirected edge kept with probability
+ * keep (keep >= 1: all kept, the reference grid2d). */
+int ggpu_gen_grid(uint32_t rows, uint32_t cols, double keep, uint64_t seed, ggpu_graph** out);
+
+/* ---- gbench helpers (src/bench/main.cpp:130-169) ---- */
+/* count distinct seeded sources with degree > 0 in the largest component
+ * (anywhere if any_component). *n_out <= count. */
+int ggpu_pick_sources(const ggpu_graph* g, uint32_t count, uint64_t seed, int any_component,
+                      uint32_t* out, uint32_t* n_out);
+
+/* ---- CPU BFS (src/cpu/bfs_cpu.cpp) ---- */
+typedef struct ggpu_cpu_bfs_opts {
+    int threads;      /* 0 = all cores */
+    int schedule;     /* cpu_omp: 0 static, 1 dynamic,64, 2 guided */
+    double alpha;     /* cpu_do */
+    double beta;
+} ggpu_cpu_bfs_opts;
+void ggpu_cpu_bfs_opts_default(ggpu_cpu_bfs_opts* o);
+/* variant: "cpu_serial" | "cpu_omp" | "cpu_do". trace may be NULL. */
+int ggpu_cpu_bfs(const ggpu_graph* g, const char* variant, uint32_t source, const ggpu_cpu_bfs_opts* opt,
+                 uint32_t* dist, ggpu_level* trace, size_t trace_cap, size_t* trace_len);
+
+/* ---- CPU PageRank (src/cpu/pagerank_cpu.cpp:20-63) ---- */
+/* in = transpose (in-neighbour lists); outdeg = out-degrees of the original
+ * graph. parallel = 0: cpu_serial, 1: cpu_omp. r_out: n doubles. */
+int ggpu_cpu_pagerank(const ggpu_graph* in, const uint32_t* outdeg, double damping, int iters, double tol,
+                      int parallel, int threads, double* r_out, int* iters_out, double* last_delta_out);
+/* acceptance (src/cpu/pagerank_cpu.hpp:37-60): L1, max rel over ref's top-1000, sum */
+void ggpu_check_ranks(uint64_t n, const double* ref, const float* got, double* l1, double* max_rel_top,
+                      double* sum);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
