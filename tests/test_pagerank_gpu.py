@@ -1,0 +1,163 @@
+"""GPU PageRank parity against the FP64 oracle.
+
+Mirrors tests/test_main.cpp:630-783: every pull variant and option extreme
+on every suite graph, fixed 20 sweeps and tol 1e-6, accepted by the
+reference's rule (L1 < 1e-4, top-1000 relative error < 1e-3, |sum-1| < 1e-4;
+src/cpu/pagerank_cpu.hpp:37-60) with the iteration count within 3 of the
+oracle's; gpu_pull_gs judged against the converged oracle
+(gsNoWorseThanJacobi, pagerank_cpu.hpp:80-84). Tolerances are the
+reference's acceptance thresholds, written in the assertions below.
+"""
+import numpy as np
+import pytest
+
+import graph_analytics_applesilicon_amd as G
+from oracle import pyoracle as O
+from tests.conftest import SUITE
+
+pytestmark = pytest.mark.gpu
+
+
+def _in_graph(name):
+    g = G.Graph.named(name)
+    t = g.transpose() if g.directed else g
+    return g, t, g.degrees()
+
+
+CFGS = [("gpu_pull_scalar", {}), ("gpu_pull_vector", {}), ("gpu_pull_adaptive", {}),
+        ("gpu_pull_adaptive", dict(small_max=1 << 30, large_min=1 << 31)),  # all lane-per-vertex
+        ("gpu_pull_adaptive", dict(small_max=0, large_min=0)),              # all workgroup-per-vertex
+        ("gpu_pull_adaptive", dict(small_max=0, large_min=1 << 30)),        # all medium
+        ("gpu_pull_adaptive", dict(small_max=2, large_min=3)),
+        ("gpu_pull_adaptive", dict(small_max=4, large_min=64, medium_lanes=8)),
+        ("gpu_pull_adaptive", dict(small_max=8, large_min=512, medium_lanes=16))]
+
+
+@pytest.mark.parametrize("name", SUITE)
+def test_pagerank_variants_match_oracle(name):
+    g, t, od = _in_graph(name)
+    pr = G.GpuPageRank(t, od)
+    for tol in (0.0, 1e-6):
+        iters = 1000 if tol > 0 else 20
+        ref, ref_iters, _ = O.pagerank(t.row_ptr, t.col_idx, od, 0.85, iters, tol)
+        for variant, kw in CFGS:
+            for tg in (64, 256, 1024):
+                res = pr.run(variant, G.pr_options(iters=iters, tol=tol, tg=tg, **kw))
+                l1, rel, s = O.check_ranks(ref, pr.ranks())
+                assert l1 < 1e-4 and rel < 1e-3 and abs(s - 1) < 1e-4, \
+                    f"{name} {variant} {kw} tol {tol} tg {tg}: L1 {l1:.3g} rel {rel:.3g} sum {s:.8f}"
+                if tol > 0 and not res.floor_stop:
+                    assert abs(res.iters - ref_iters) <= 3, f"{name} {variant}: {res.iters} vs {ref_iters}"
+                if tol == 0:
+                    assert res.iters == 20
+                    assert abs(res.sum - 1.0) < 1e-4  # FP64 sum of the FP32 ranks
+
+
+def test_pagerank_known_answers_on_gpu():
+    """Closed forms of tests/test_main.cpp:630-652 on the GPU (FP32, to 1e-6)."""
+    cases = [(2, [(0, 1)], [0.5 / 1.425, 1 - 0.5 / 1.425]),
+             (3, [(0, 1), (1, 2), (2, 0)], [1 / 3] * 3)]
+    for n, edges, want in cases:
+        g = G.Graph.from_edges(n, edges, directed=True)
+        t = g.transpose()
+        pr = G.GpuPageRank(t, g.degrees())
+        for v in ("gpu_pull_scalar", "gpu_pull_vector", "gpu_pull_adaptive", "gpu_pull_gs"):
+            pr.run(v, G.pr_options(iters=200))
+            assert np.allclose(pr.ranks(), want, atol=1e-6), v
+    for name, n in (("complete200", 200), ("empty1000", 1000)):
+        g = G.Graph.named(name)
+        pr = G.GpuPageRank(g, g.degrees())
+        pr.run("gpu_pull_adaptive", G.pr_options(iters=100))
+        assert np.allclose(pr.ranks(), 1.0 / n, rtol=1e-5)
+
+
+def test_pagerank_gs_converges_to_oracle():
+    colored = natural = deferred_graphs = 0
+    failures = []
+    for name in SUITE:
+        g, t, od = _in_graph(name)
+        if g.n == 0:
+            continue
+        conv, _, _ = O.pagerank(t.row_ptr, t.col_idx, od, 0.85, 1000, 1e-10)
+        jac, _, _ = O.pagerank(t.row_ptr, t.col_idx, od, 0.85, 1000, 1e-6)
+        jerr = _check64(conv, jac)  # the Jacobi reference's own distance from convergence
+        for K in (0, 2):
+            pr = G.GpuPageRank(t, od)
+            pr.prepare_gs(K)
+            is_col, sets, dfr = pr.gs_info()
+            colored += is_col
+            natural += not is_col
+            deferred_graphs += dfr > 0
+            for tg in (64, 256, 1024):
+                for what, tol, tol_rel, iters in (("20 sweeps", 0.0, 0.0, 20), ("200 sweeps", 0.0, 0.0, 200),
+                                                  ("tol 2.5e-7", 2.5e-7, 0.0, 1000),
+                                                  ("tol 1e-6 + rel", 1e-6, 2.5e-5, 1000)):
+                    res = pr.run("gpu_pull_gs", G.pr_options(iters=iters, tol=tol, tol_rel=tol_rel, tg=tg))
+                    c = O.check_ranks(conv, pr.ranks())
+                    if iters == 20:
+                        ok = abs(c[2] - 1.0) < 1e-4
+                    elif iters == 200:
+                        ok = O.ranks_ok(*c)
+                    else:
+                        ok = O.gs_no_worse(c, jerr)
+                    if not ok:
+                        failures.append((name, K, tg, what, c, res.iters, jerr))
+    assert not failures, failures[:5]
+    assert colored > 0 and natural > 0 and deferred_graphs > 0
+
+
+def _check64(ref, got):
+    l1 = float(np.abs(ref - got).sum())
+    top = np.argsort(-ref, kind="stable")[: min(1000, len(ref))]
+    rel = float((np.abs(ref[top] - got[top]) / ref[top]).max()) if len(top) else 0.0
+    return l1, rel, float(got.sum())
+
+
+@pytest.mark.parametrize("name", ["rmat12_directed", "grid100x100", "star100k", "two_components"])
+def test_inplace_jacobi_variants_converge(name):
+    g, t, od = _in_graph(name)
+    conv, _, _ = O.pagerank(t.row_ptr, t.col_idx, od, 0.85, 1000, 1e-10)
+    pr = G.GpuPageRank(t, od)
+    for v in ("gpu_pull_scalar", "gpu_pull_vector", "gpu_pull_adaptive"):
+        pr.run(v, G.pr_options(iters=200, in_place=1))
+        c = O.check_ranks(conv, pr.ranks())
+        assert O.ranks_ok(*c), (v, c)
+
+
+def test_pr_errors_are_reported():
+    g, t, od = _in_graph("rmat12_directed")
+    pr = G.GpuPageRank(t, od)
+    with pytest.raises(RuntimeError):
+        pr.ranks()  # no run yet
+    with pytest.raises(ValueError):
+        pr.run("gpu_pull_adaptive", G.pr_options(tg=100))
+    with pytest.raises(ValueError):
+        pr.run("gpu_pull_adaptive", G.pr_options(medium_lanes=3))
+
+
+@pytest.mark.parametrize("name", ["rmat12_directed", "grid100x100", "two_components", "path1000"])
+def test_ppr_block_matches_oracle(name):
+    g, t, od = _in_graph(name)
+    pr = G.GpuPageRank(t, od)
+    rng = np.random.default_rng(3)
+    for B in (1, 5, 64, 100):
+        seeds = rng.integers(0, g.n, size=B).astype(np.uint32)
+        pr.run_ppr(seeds, G.pr_options(iters=60))
+        blk = pr.ppr_block()
+        assert blk.shape == (g.n, B)
+        for b in range(B):
+            ref, _, _ = O.ppr(t.row_ptr, t.col_idx, od, int(seeds[b]), 0.85, 60)
+            err = float(np.abs(blk[:, b].astype(np.float64) - ref).sum())
+            assert err < 2e-5, f"{name} B={B} col {b} seed {seeds[b]}: L1 {err:.3g}"
+            assert abs(float(blk[:, b].astype(np.float64).sum()) - 1.0) < 2e-5
+
+
+def test_ppr_isolated_seed_is_one_hot():
+    g = G.Graph.named("two_components")
+    iso = G.Graph.from_edges(5, [(0, 1), (1, 2)], directed=False)  # 3, 4 isolated
+    pr = G.GpuPageRank(iso, iso.degrees())
+    pr.run_ppr([3, 4, 0], G.pr_options(iters=30))
+    blk = pr.ppr_block()
+    assert np.allclose(blk[:, 0], [0, 0, 0, 1, 0]) and np.allclose(blk[:, 1], [0, 0, 0, 0, 1])
+    assert abs(blk[:, 2].sum() - 1) < 1e-6
+    del g
