@@ -1,0 +1,264 @@
+#!/usr/bin/env python3
+"""Benchmark of the BFS / PageRank hot path on one MI355X.
+
+Workload (the scale-22 RMAT graph of the benchmark configurations): undirected
+RMAT, scale 22, a/b/c/d = .57/.19/.19/.05, 8 generated edges per vertex,
+deduplicated, self-loops removed, symmetrized, ids permuted from seed 2;
+64 seeded sources in the largest component (pickSources, seed 2).
+
+One step = one batch of the 64 sources through BFS on the GPU, inputs
+already in HBM. `value` is the aggregate GTEPS of the batch (sum over sources
+of the component's undirected edges / batch time). Beside it the line carries
+the plain per-source gpu_do rate, PageRank time to L1 < 1e-4 against the FP64
+oracle, the roofline of the dominant kernel (HIP events on the library's
+stream) and the reference's own cpu_do / cpu_omp timed on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-pagerank]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--sources", type=int, default=64)
+    ap.add_argument("--graph", default="rmat22_c2")
+    ap.add_argument("--no-pagerank", action="store_true")
+    ap.add_argument("--pr-graph", default="rmat23_c5",
+                    help="graph of the PageRank part (default: the scale-23 heavy-skew RMAT)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", default="auto", choices=["auto", "batched", "sequential"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build runs on a single "
+              "MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import graph_analytics_applesilicon_amd as G
+    from oracle import pyoracle as O
+
+    t0 = time.time()
+    g = G.Graph.named(args.graph)
+    n, m = g.n, g.m
+    sources = g.pick_sources(args.sources, 2)
+    lcc, _ = g.largest_component()
+    comp_edges = lcc.m // 2  # every source lies in the largest component
+    log(f"graph {args.graph}: n {n} m {m} (lcc undirected edges {comp_edges}), {len(sources)} sources, "
+        f"generated in {time.time() - t0:.1f}s")
+    del lcc
+
+    ctx = G.default_context()
+    name, cus, hbm = ctx.info()
+    bfs = G.GpuBfs(g, ctx)
+    rp, ci = g.row_ptr, g.col_idx
+    out = {}
+
+    # ---------------------------------------------------------- plain gpu_do
+    plain_opts = G.bfs_options(time_levels=0)
+    for s in sources[:2]:
+        bfs.run("gpu_do", int(s), plain_opts, trace=False, fetch=False)
+    t = time.perf_counter()
+    for s in sources:
+        bfs.run("gpu_do", int(s), plain_opts, trace=False, fetch=False)
+    plain_s = time.perf_counter() - t
+    plain_gteps = len(sources) * comp_edges / plain_s / 1e9
+    log(f"plain gpu_do: {len(sources)} sequential traversals {plain_s * 1e3:.2f} ms, {plain_gteps:.2f} GTEPS")
+    # parity of the plain path on a few sources
+    bad = 0
+    for s in sources[:4]:
+        d = bfs.run("gpu_do", int(s), plain_opts, trace=False).dist
+        bad += int(not np.array_equal(d, O.bfs_serial(rp, ci, int(s))))
+    out["plain_gpu_do"] = {"gteps": plain_gteps, "ms_per_64": plain_s * 1e3 * 64 / len(sources),
+                           "bit_exact_sources_checked": 4, "mismatches": bad}
+
+    # ---------------------------------------------------------- timed steps
+    batched = args.mode in ("auto", "batched") and hasattr(ctx.lib, "ggpu_msbfs_run")
+    bopts = G.bfs_options(policy=G.POLICY_LEARNED) if batched else plain_opts
+    if batched:
+        pol = G.load_policy_file()
+        if pol is not None:
+            G.policy_set(pol, ctx)
+
+    def step():
+        if batched:
+            _, res, _ = bfs.run_batch(sources, bopts, fetch=False)
+            return res.gpu_ms
+        gms = 0.0
+        for s in sources:
+            bfs.run("gpu_do", int(s), plain_opts, trace=False, fetch=False)
+        return gms
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    t = time.perf_counter()
+    kernel_ms = []
+    for _ in range(args.steps):
+        kernel_ms.append(step())
+    ctx.sync()
+    total_s = time.perf_counter() - t
+    ms_per_step = total_s * 1e3 / args.steps
+    value = len(sources) * comp_edges / (ms_per_step / 1e3) / 1e9
+    log(f"timed: {args.steps} steps, {ms_per_step:.3f} ms/step, {value:.2f} GTEPS ({'batched' if batched else 'sequential gpu_do'})")
+
+    # parity of the measured path
+    if batched:
+        dist, res, _ = bfs.run_batch(sources, bopts, fetch=True)
+        mism = 0
+        for i in range(0, len(sources), max(1, len(sources) // 8)):
+            mism += int(not np.array_equal(dist[i], O.bfs_serial(rp, ci, int(sources[i]))))
+        out["parity"] = {"sources_checked": len(range(0, len(sources), max(1, len(sources) // 8))),
+                         "mismatches": mism}
+
+    # ---------------------------------------------------------- roofline
+    roof = None
+    if batched:
+        # one persistent launch = one batch: 4(n+1) + 4m + 4*n*S algorithmic bytes
+        bytes_batch = 4 * (n + 1) + 4 * m + 4 * n * len(sources)
+        avg_ms = float(np.mean(kernel_ms))
+        ach = bytes_batch / (avg_ms / 1e3) / 1e9
+        roof = {"kernel": "k_msbfs (persistent batch)", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "bytes_per_launch": bytes_batch, "avg_launch_ms": avg_ms}
+
+    # ---------------------------------------------------------- PageRank
+    if not args.no_pagerank:
+        del bfs
+        pg = g if args.pr_graph == args.graph else G.Graph.named(args.pr_graph)
+        out["pagerank"] = pagerank_part(G, O, pg, ctx)
+        out["pagerank"]["graph"] = args.pr_graph
+        if roof is None:
+            roof = out["pagerank"].pop("roofline")
+        else:
+            out["pagerank_roofline"] = out["pagerank"].pop("roofline")
+
+    # ---------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(O, g, sources, comp_edges)
+
+    line = {
+        "metric": "BFS GTEPS batched+learned vs plain gpu_do; PageRank ms to L1<1e-4 vs FP64",
+        "value": value,
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.graph}: undirected RMAT scale 22, ef 8, a/b/c .57/.19/.19, ids permuted "
+                               f"(seed 2); {len(sources)} seeded LCC sources per step",
+                   "n": n, "m_arcs": m, "sources": len(sources),
+                   "path": "batched MS-BFS (one persistent launch)" if batched else "64 sequential gpu_do",
+                   "device": name, "cus": cus},
+        "speedup_vs_plain_gpu_do": value / plain_gteps if plain_gteps else None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    line.update(out)
+    print(json.dumps(line), flush=True)
+
+
+def pagerank_part(G, O, g, ctx):
+    """Time to L1 < 1e-4 against the FP64 oracle, and the per-sweep roofline."""
+    n, m = g.n, g.m
+    od = g.degrees()
+    t = g if not g.directed else g.transpose()
+    pr = G.GpuPageRank(t, od, ctx)  # undirected: the graph is its own transpose
+    ref = None
+    rlib = O.ref_lib()
+    t0 = time.time()
+    if rlib is not None:
+        ref, it, _ = O.ref_pagerank(t.row_ptr, t.col_idx, od, 0.85, 1000, 1e-10, parallel=True)
+    log(f"FP64 converged oracle (the reference's pagerankOmp, tol 1e-10): {time.time() - t0:.1f}s")
+    res = {}
+    bytes_sweep = 4 * (n + 1) + 8 * m + 4 * n
+    # per-sweep time of the fused Jacobi sweep in each vertex order (20 sweeps, no host sync)
+    sweeps = {}
+    for order, name in ((G.PR_ORDER_GIVEN, "given"), (G.PR_ORDER_DEGREE, "degree")):
+        fo = G.pr_options(iters=20, order=order)
+        pr.run("gpu_pull_adaptive", fo)
+        r = pr.run("gpu_pull_adaptive", fo)
+        sweeps[name] = r.gpu_ms / 20
+        log(f"  sweep ({name} order): {sweeps[name] * 1e3:.1f} us")
+    best_name = min(sweeps, key=sweeps.get)
+    order = G.PR_ORDER_DEGREE if best_name == "degree" else G.PR_ORDER_GIVEN
+    res["sweep_ms"] = sweeps
+    # Jacobi, certified stop: ||r_k - r*||_1 <= d/(1-d) * delta_k  =>  delta_k < 1e-4 * (1-d)/d
+    tol_cert = 1e-4 * 0.15 / 0.85
+    for variant, opts, key in (("gpu_pull_adaptive", G.pr_options(tol=tol_cert, iters=1000, order=order),
+                                "jacobi_certified"),
+                               ("gpu_pull_gs", G.pr_options(tol=1e-6, iters=1000, order=order), "gs_tol_1e-6")):
+        pr.run(variant, opts)  # warm-up (builds the order)
+        t1 = time.perf_counter()
+        r = pr.run(variant, opts)
+        ms = (time.perf_counter() - t1) * 1e3
+        ent = {"ms": ms, "iters": r.iters, "gpu_ms": r.gpu_ms, "last_delta": r.last_delta, "order": best_name}
+        if ref is not None:
+            l1, rel, s = O.check_ranks(ref, pr.ranks())
+            ent.update({"l1_vs_fp64": l1, "max_rel_top1000": rel, "sum": s, "l1_below_1e-4": bool(l1 < 1e-4)})
+        res[key] = ent
+    per_sweep_ms = sweeps[best_name]
+    ach = bytes_sweep / (per_sweep_ms / 1e3) / 1e9
+    res["roofline"] = {"kernel": f"k_pr_sweep (gpu_pull_adaptive, Jacobi, stream layout, {best_name} order)",
+                       "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": ach / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bytes_sweep,
+                       "avg_launch_ms": per_sweep_ms}
+    log(f"pagerank: certified Jacobi {res['jacobi_certified']['ms']:.2f} ms "
+        f"({res['jacobi_certified']['iters']} it), GS {res['gs_tol_1e-6']['ms']:.2f} ms "
+        f"({res['gs_tol_1e-6']['iters']} it), sweep {per_sweep_ms * 1e3:.1f} us = {ach:.0f} GB/s")
+    return res
+
+
+def cpu_baseline(O, g, sources, comp_edges):
+    """The reference's cpu_do (OpenMP, all host cores) on a bounded sample of the sources."""
+    rlib = O.ref_lib()
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    rp, ci = g.row_ptr, g.col_idx
+    kind = "reference" if rlib is not None else "port"
+    times = []
+    budget = 15.0
+    t_start = time.time()
+    for s in sources:
+        t = time.perf_counter()
+        if rlib is not None:
+            O.ref_bfs_cpu(rp, ci, int(s), "cpu_do", threads=threads, trace=False)
+        else:
+            O.bfs_do_levels(rp, ci, int(s))
+        times.append(time.perf_counter() - t)
+        if time.time() - t_start > budget:
+            break
+    per = float(np.median(times))
+    return {"value": comp_edges / per / 1e9, "unit": "GTEPS", "cores": threads if rlib is not None else 1,
+            "kind": kind,
+            "sample": f"cpu_do (alpha 15, beta 18) from {len(times)} of the {len(sources)} sources, median "
+                      f"{per * 1e3:.1f} ms per traversal"}
+
+
+if __name__ == "__main__":
+    main()

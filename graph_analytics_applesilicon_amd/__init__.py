@@ -38,6 +38,8 @@ LOOP_HOST, LOOP_GRAPH, LOOP_PERSISTENT = 0, 1, 2
 POLICY_HEURISTIC, POLICY_LEARNED, POLICY_SCRIPTED = 0, 1, 2
 ACTION_TD_EDGE, ACTION_TD_BINNED, ACTION_BU = 0, 1, 2
 PR_SCALAR, PR_VECTOR, PR_ADAPTIVE, PR_GS = 1, 2, 3, 4
+PR_LAYOUT_STREAM, PR_LAYOUT_BINS = 0, 1
+PR_ORDER_GIVEN, PR_ORDER_DEGREE = 0, 1
 PR_VARIANTS = {"gpu_pull_scalar": PR_SCALAR, "gpu_pull_vector": PR_VECTOR,
                "gpu_pull_adaptive": PR_ADAPTIVE, "gpu_pull_gs": PR_GS}
 MLP_PARAMS = 1347
@@ -74,7 +76,8 @@ class PrOpts(C.Structure):
     """ggpu_pr_opts = gpu::GpuPrOptions (src/metal/pagerank_gpu.hpp:27-36) + medium_lanes."""
     _fields_ = [("damping", C.c_float), ("iters", C.c_int), ("tol", C.c_double), ("tol_rel", C.c_double),
                 ("tg", C.c_uint32), ("small_max", C.c_uint32), ("large_min", C.c_uint32),
-                ("in_place", C.c_int), ("medium_lanes", C.c_uint32)]
+                ("in_place", C.c_int), ("medium_lanes", C.c_uint32), ("layout", C.c_uint32),
+                ("order", C.c_uint32)]
 
 
 class PrResult(C.Structure):

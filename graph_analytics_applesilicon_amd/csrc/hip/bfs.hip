@@ -44,6 +44,9 @@ __global__ void k_td_binned(BfsDev d, uint32_t cur, uint32_t level, uint32_t sma
 __global__ void k_bu(BfsDev d, uint32_t cur, uint32_t level, unsigned long long* stats) {
     ph_bu(d, d.bm[cur], d.bm[cur ^ 1], level, stats, blockIdx.x, gridDim.x);
 }
+__global__ void k_bu_heavy(BfsDev d, uint32_t cur, uint32_t level, unsigned long long* stats) {
+    ph_bu_heavy(d, d.bm[cur], d.bm[cur ^ 1], level, stats, blockIdx.x, gridDim.x);
+}
 __global__ void k_dist_to_bitmap(BfsDev d, uint32_t cur, uint32_t level) {
     ph_dist_to_bitmap(d, d.bm[cur], level, blockIdx.x, gridDim.x);
 }
@@ -98,7 +101,10 @@ BfsDev BfsEngine::dev() {
     d.visited = bms.p ? bms.p + 2ull * nw : nullptr;
     d.tile = tile.p;
     d.stats = stats.p;
-    d.conv = stats.p + kStatSlots * kStatWords;
+    d.conv = stats.p + kSlotWords;
+    d.heavy = heavy.p;
+    d.n_heavy = n_heavy;
+    d.heavy_min = kHeavyMin;
     d.n = n;
     d.nw = nw;
     d.m = m;
@@ -115,11 +121,28 @@ BfsEngine::BfsEngine(ggpu_ctx* c, uint64_t n_, uint64_t m_, const uint32_t* hrp,
     rp.alloc(n + 1);
     ci.alloc(m ? m : 1);
     dist.alloc(n ? n : 1);
-    stats.alloc(kStatSlots * kStatWords + 8);
+    stats.alloc(kStatSlots * kSlotWords + kConvWords);
     GGPU_HIP_TRY(hipMemcpyAsync(rp.p, hrp, 4ull * (n + 1), hipMemcpyHostToDevice, ctx->stream));
     if (m) GGPU_HIP_TRY(hipMemcpyAsync(ci.p, hci, 4ull * m, hipMemcpyHostToDevice, ctx->stream));
     ensure_edge_buffers();
     ensure_bitmaps();
+    // Bottom-up heavy items: vertices of degree >= kHeavyMin, lists longer than
+    // kHeavyChunk split into chunks; longest first so the tail stays short.
+    {
+        std::vector<uint4> items;
+        for (uint32_t v = 0; v < n; ++v) {
+            const uint32_t dg = host.deg(v);
+            if (dg < kHeavyMin) continue;
+            for (uint32_t x = host.rp[v]; x < host.rp[v + 1]; x += kHeavyChunk)
+                items.push_back(make_uint4(v, x, std::min(x + kHeavyChunk, host.rp[v + 1]), dg));
+        }
+        std::stable_sort(items.begin(), items.end(), [](const uint4& a, const uint4& b) { return a.w > b.w; });
+        n_heavy = (uint32_t)items.size();
+        heavy.alloc(items.size() + 1);
+        if (!items.empty())
+            GGPU_HIP_TRY(hipMemcpyAsync(heavy.p, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                                        ctx->stream));
+    }
     GGPU_HIP_TRY(hipStreamSynchronize(ctx->stream));
     int dev_blocks = 0;
     GGPU_HIP_TRY(hipDeviceGetAttribute(&dev_blocks, hipDeviceAttributeMultiprocessorCount, ctx->device));
@@ -187,18 +210,26 @@ void BfsEngine::host_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, std
             where = ON_DEV;
             if (timing) GGPU_HIP_TRY(hipEventRecord(event(ev), s));
             unsigned long long* slot = stats.p;
-            GGPU_HIP_TRY(hipMemsetAsync(slot, 0, kStatWords * sizeof(unsigned long long), s));
+            const size_t words = kSlotWords + kConvWords;  // striped counters, then append tails
+            GGPU_HIP_TRY(hipMemsetAsync(slot, 0, words * sizeof(unsigned long long), s));
             launch_level(d, st, o, slot, tg);
             if (timing) GGPU_HIP_TRY(hipEventRecord(event(ev + 1), s));
             ev += timing ? 2 : 0;
             auto* pin = (unsigned long long*)ctx_pinned(ctx, 4096);
-            GGPU_HIP_TRY(hipMemcpyAsync(pin, slot, kStatWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+            GGPU_HIP_TRY(hipMemcpyAsync(pin, slot, words * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
             GGPU_HIP_TRY(hipStreamSynchronize(s));
-            r.found = st.action == GGPU_ACTION_TD_BINNED ? pin[S_BIN0] + pin[S_BIN0 + 1] + pin[S_BIN0 + 2]
-                                                         : pin[S_FOUND];
-            r.examined = bu ? pin[S_EXAMINED] : st.total;
-            r.next_edges = pin[S_NEXT_EDGES];
-            for (int k = 0; k < 3; ++k) r.bins[k] = pin[S_BIN0 + k];
+            unsigned long long sum[kStatWords] = {0};
+            for (int t = 0; t < kStripes; ++t)
+                for (int k = 0; k < kStatWords; ++k) sum[k] += pin[t * kStatWords + k];
+            const unsigned long long* conv = pin + kSlotWords;
+            if (st.action == GGPU_ACTION_TD_BINNED) {
+                for (int k = 0; k < 3; ++k) r.bins[k] = conv[kTailBin0 + k];
+                r.found = r.bins[0] + r.bins[1] + r.bins[2];
+            } else {
+                r.found = bu ? sum[S_FOUND] : conv[kTailQueue];
+            }
+            r.examined = bu ? sum[S_EXAMINED] : st.total;
+            r.next_edges = sum[S_NEXT_EDGES];
         }
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (trace) trace->push_back(trace_record(st, r, cpu ? 'C' : 'G', ms));
@@ -232,11 +263,14 @@ void BfsEngine::launch_level(BfsDev& d, LoopState& st, const ggpu_bfs_opts& o, u
         }
         k_bu<<<blocks_for(nw, tg / kWave, grid_cap), tg, 0, s>>>(d, cur, level, slot);
         GGPU_HIP_TRY(hipGetLastError());
+        if (n_heavy) {
+            k_bu_heavy<<<blocks_for(n_heavy, tg / kWave, grid_cap), tg, 0, s>>>(d, cur, level, slot);
+            GGPU_HIP_TRY(hipGetLastError());
+        }
         return;
     }
     if (act == GGPU_ACTION_TD_EDGE) {
         if (st.fmt != FMT_QUEUE) {
-            GGPU_HIP_TRY(hipMemsetAsync(d.conv, 0, 4 * sizeof(unsigned long long), s));
             if (st.fmt == FMT_BITMAP)
                 k_bitmap_to_queue<<<blocks_for(nw, tg, grid_cap), tg, 0, s>>>(d, cur, cur);
             else
@@ -256,7 +290,6 @@ void BfsEngine::launch_level(BfsDev& d, LoopState& st, const ggpu_bfs_opts& o, u
     }
     // binned top-down
     if (st.fmt != FMT_BINS) {
-        GGPU_HIP_TRY(hipMemsetAsync(d.conv, 0, 4 * sizeof(unsigned long long), s));
         k_dist_collect<<<blocks_for(n, tg, grid_cap), tg, 0, s>>>(d, level, 1, cur, o.small_max, o.large_min);
         GGPU_HIP_TRY(hipGetLastError());
         // bin sizes of a collected frontier: count on the host from the level's frontier size split

@@ -145,6 +145,9 @@ __host__ __device__ inline void fold_level(LoopState& st, const LevelOut& r, con
     if (st.size == 0) st.done = 1;
 }
 
+constexpr uint32_t kHeavyMin = 32;     // bottom-up: degree handled by a wave instead of a lane
+constexpr uint32_t kHeavyChunk = 4096; // bottom-up: edges per heavy item
+
 struct HybridMirror {
     std::vector<uint32_t> dist;
     std::vector<uint32_t> q[2], deg[2];
@@ -160,6 +163,8 @@ struct BfsEngine {
     Csr host;  // host copy of the graph: degrees, hybrid CPU levels, policy features
     DBuf<uint32_t> rp, ci, dist, qa, qb, dega, degb, tile, bins;
     DBuf<uint64_t> bms;
+    DBuf<uint4> heavy;       // bottom-up heavy items
+    uint32_t n_heavy = 0;
     DBuf<unsigned long long> stats;
     std::vector<hipEvent_t> events;
     double last_gpu_ms = 0;

@@ -19,6 +19,19 @@ namespace ggpu {
 
 enum StatWord : int { S_FRONTIER = 0, S_EXAMINED = 1, S_FOUND = 2, S_NEXT_EDGES = 3, S_BIN0 = 4 };
 constexpr int kStatWords = 8;
+// Level counters are striped: block b adds its totals (reduced in registers
+// and LDS first) to stripe b % kStripes, and readers sum the stripes. One
+// device-scope atomic per block per counter, spread over kStripes addresses,
+// instead of one per wave on a single address (same-address atomics
+// serialize across the XCDs). Queue tails stay single counters but are
+// reserved once per block and round.
+constexpr int kStripes = 32;
+constexpr int kSlotWords = kStripes * kStatWords;
+// conv[]: append counters. [0] collect-to-queue, [1..3] collect-to-bins,
+// [kTailQueue] edge-expand queue tail, [kTailBin0..+2] binned queue tails.
+constexpr int kTailQueue = 4;
+constexpr int kTailBin0 = 5;
+constexpr int kConvWords = 8;
 constexpr int kStatSlots = 4;     // ring of per-level stats slots
 constexpr int kOwnerStage = 2048; // LDS-staged owner range of the edge expand
 constexpr int kScanItems = 16;    // elements per thread of the tile scan
@@ -38,6 +51,9 @@ struct BfsDev {
     uint32_t* tile;        // scan tile sums
     unsigned long long* stats;  // kStatSlots x kStatWords
     unsigned long long* conv;   // conversion append counters [4]
+    const uint4* heavy;    // bottom-up heavy items {v, first edge, end edge, degree}
+    uint32_t n_heavy;
+    uint32_t heavy_min;    // degree from which a vertex is scanned by the heavy pass
     uint32_t n;
     uint32_t nw;           // bitmap words
     uint64_t m;
@@ -45,6 +61,48 @@ struct BfsDev {
 
 // dist only moves from INF to a final value, so a plain (possibly stale) load
 // that sees a value is exact; a stale INF just costs a failing CAS.
+// Adds per-thread counter values, block-reduced, to the block's stripe.
+template <int K>
+__device__ __forceinline__ void block_flush(unsigned long long* slot, const int (&word)[K],
+                                            const unsigned long long (&val)[K]) {
+    __shared__ unsigned long long s_red[K][16];
+    const uint32_t lane = lane_id(), w = threadIdx.x / kWave, nwv = blockDim.x / kWave;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const unsigned long long t = wave_sum(val[k]);
+        if (lane == 0) s_red[k][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        unsigned long long t = 0;
+        for (uint32_t i = 0; i < nwv; ++i) t += s_red[threadIdx.x][i];
+        if (t) atomicAdd(&slot[(blockIdx.x % kStripes) * kStatWords + word[threadIdx.x]], t);
+    }
+    __syncthreads();
+}
+
+// Block-wide append: every thread of the block calls it (uniform control
+// flow); one atomic per block on `counter`. Returns the slot of threads with
+// `want` set.
+__device__ __forceinline__ unsigned long long block_append(bool want, unsigned long long* counter) {
+    __shared__ uint32_t s_cnt[16];
+    __shared__ unsigned long long s_base[17];
+    const uint32_t lane = lane_id(), w = threadIdx.x / kWave, nwv = blockDim.x / kWave;
+    const uint64_t mask = __ballot(want);
+    if (lane == 0) s_cnt[w] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (uint32_t i = 0; i < nwv; ++i) {
+            s_base[1 + i] = run;
+            run += s_cnt[i];
+        }
+        s_base[0] = run ? atomicAdd(counter, run) : 0ull;
+    }
+    __syncthreads();
+    return s_base[0] + s_base[1 + w] + rank_in(mask);
+}
+
 __device__ __forceinline__ bool try_claim(uint32_t* dist, uint32_t u, uint32_t value) {
     if (dist[u] != GGPU_INF) return false;
     return atomicCAS(&dist[u], GGPU_INF, value) == GGPU_INF;
@@ -141,6 +199,7 @@ __device__ __forceinline__ void ph_td_edge(const BfsDev& d, uint32_t cur, uint32
     uint32_t* nq = d.q[cur ^ 1];
     uint32_t* ndeg = d.deg[cur ^ 1];
     const uint32_t chunk = blockDim.x * ept;
+    unsigned long long next_edges = 0;
     for (uint64_t c0l = (uint64_t)block * chunk; c0l < total; c0l += (uint64_t)grid * chunk) {
         const uint32_t c0 = (uint32_t)c0l;
         const uint32_t c1 = (uint32_t)min<uint64_t>(c0l + chunk, total);
@@ -178,22 +237,24 @@ __device__ __forceinline__ void ph_td_edge(const BfsDev& d, uint32_t cur, uint32
                 }
             }
             const bool claimed = has && try_claim(d.dist, u, level + 1);
-            if (__ballot(claimed) == 0) continue;  // wave-uniform
             const uint32_t udeg = claimed ? d.rp[u + 1] - d.rp[u] : 0u;
-            const unsigned long long pos = wave_append(claimed, &stats[S_FOUND]);
+            const unsigned long long pos = block_append(claimed, &d.conv[kTailQueue]);
             if (claimed) {
                 nq[pos] = u;
                 ndeg[pos] = udeg;
             }
-            wave_add(&stats[S_NEXT_EDGES], udeg);
+            next_edges += udeg;
         }
         __syncthreads();  // LDS reuse by the next chunk
     }
+    const int words[1] = {S_NEXT_EDGES};
+    const unsigned long long vals[1] = {next_edges};
+    block_flush<1>(stats, words, vals);
 }
 
 // ---------------------------------------------------------- binned top-down
 __device__ __forceinline__ void bin_visit(const BfsDev& d, uint32_t nxt, bool has, uint32_t e, uint32_t level,
-                                          uint32_t small_max, uint32_t large_min, unsigned long long* stats) {
+                                          uint32_t small_max, uint32_t large_min, unsigned long long& next_edges) {
     const uint32_t u = has ? d.ci[e] : 0u;
     const bool claimed = has && try_claim(d.dist, u, level + 1);
     if (__ballot(claimed) == 0) return;  // wave-uniform
@@ -202,10 +263,10 @@ __device__ __forceinline__ void bin_visit(const BfsDev& d, uint32_t nxt, bool ha
 #pragma unroll
     for (uint32_t k = 0; k < 3; ++k) {
         const bool mine = claimed && b == k;
-        const unsigned long long pos = wave_append(mine, &stats[S_BIN0 + k]);
+        const unsigned long long pos = wave_append(mine, &d.conv[kTailBin0 + k]);
         if (mine) d.bins[nxt][k][pos] = u;
     }
-    wave_add(&stats[S_NEXT_EDGES], udeg);
+    next_edges += udeg;
 }
 
 // Bin counts cnt[3]. All blocks stride over the large bin (workgroup per
@@ -216,18 +277,19 @@ __device__ __forceinline__ void ph_td_binned(const BfsDev& d, uint32_t cur, uint
                                              unsigned long long* stats, uint32_t block, uint32_t grid) {
     const uint32_t nxt = cur ^ 1;
     const uint32_t lane = lane_id();
+    unsigned long long next_edges = 0;
     for (uint32_t i = block; i < cnt2; i += grid) {
         const uint32_t v = d.bins[cur][2][i];
         const uint32_t b = d.rp[v], e = d.rp[v + 1];
         for (uint32_t base = b; base < e; base += blockDim.x)
-            bin_visit(d, nxt, base + threadIdx.x < e, base + threadIdx.x, level, small_max, large_min, stats);
+            bin_visit(d, nxt, base + threadIdx.x < e, base + threadIdx.x, level, small_max, large_min, next_edges);
     }
     const uint32_t wpb = blockDim.x / kWave, wid = threadIdx.x / kWave;
     for (uint32_t i = block * wpb + wid; i < cnt1; i += grid * wpb) {
         const uint32_t v = d.bins[cur][1][i];
         const uint32_t b = d.rp[v], e = d.rp[v + 1];
         for (uint32_t base = b; base < e; base += kWave)
-            bin_visit(d, nxt, base + lane < e, base + lane, level, small_max, large_min, stats);
+            bin_visit(d, nxt, base + lane < e, base + lane, level, small_max, large_min, next_edges);
     }
     for (uint32_t w0 = (block * wpb + wid) * kWave; w0 < cnt0; w0 += grid * blockDim.x) {
         const uint32_t idx = w0 + lane;
@@ -249,47 +311,107 @@ __device__ __forceinline__ void ph_td_binned(const BfsDev& d, uint32_t cur, uint
             }
             const uint32_t ob = __shfl(begin, (int)owner, kWave);
             const uint32_t op = __shfl(pre, (int)owner, kWave);
-            bin_visit(d, nxt, j < tot, ob + (j - op), level, small_max, large_min, stats);
+            bin_visit(d, nxt, j < tot, ob + (j - op), level, small_max, large_min, next_edges);
         }
     }
+    const int words[1] = {S_NEXT_EDGES};
+    const unsigned long long vals[1] = {next_edges};
+    block_flush<1>(stats, words, vals);
 }
 
 // ---------------------------------------------------------------- bottom-up
-// Lane v%64 of the wave owning word v/64. `visited` marks vertices with a
-// final distance; the wave writes next[w] and visited[w] whole (no atomics).
+// Light pass: lane v%64 of the wave owning word v/64, for vertices of degree
+// < heavy_min (a short serial scan). `visited` marks vertices with a final
+// distance; the wave writes next[w] and visited[w] whole (no atomics).
+// Heavier vertices are left to ph_bu_heavy, which runs after this pass.
 __device__ __forceinline__ void ph_bu(const BfsDev& d, const uint64_t* front, uint64_t* next, uint32_t level,
                                       unsigned long long* stats, uint32_t block, uint32_t grid) {
     const uint32_t lane = lane_id();
     const uint32_t wpb = blockDim.x / kWave, wid = threadIdx.x / kWave;
+    unsigned long long t_found = 0, t_examined = 0, t_next = 0;
     for (uint32_t w = block * wpb + wid; w < d.nw; w += grid * wpb) {
         const uint64_t vis = d.visited[w];
         const uint32_t v = w * kWave + lane;
         bool found = false;
         uint32_t examined = 0, ndeg = 0;
-        if (v < d.n && !((vis >> lane) & 1ull)) {
+        if (vis != ~0ull && v < d.n && !((vis >> lane) & 1ull)) {
             const uint32_t b = d.rp[v], e = d.rp[v + 1];
-            for (uint32_t x = b; x < e; ++x) {
-                ++examined;
-                const uint32_t u = d.ci[x];
-                if ((front[u >> 6] >> (u & 63)) & 1ull) {
-                    found = true;
-                    break;
+            if (e - b < d.heavy_min) {
+                for (uint32_t x = b; x < e; ++x) {
+                    ++examined;
+                    const uint32_t u = d.ci[x];
+                    if ((front[u >> 6] >> (u & 63)) & 1ull) {
+                        found = true;
+                        break;
+                    }
                 }
-            }
-            if (found) {
-                d.dist[v] = level + 1;
-                ndeg = e - b;
+                if (found) {
+                    d.dist[v] = level + 1;
+                    ndeg = e - b;
+                }
             }
         }
         const uint64_t mask = __ballot(found);
         if (lane == 0) {
             next[w] = mask;
-            d.visited[w] = vis | mask;
-            if (mask) atomicAdd(&stats[S_FOUND], (unsigned long long)__popcll(mask));
+            if (mask) d.visited[w] = vis | mask;
         }
-        wave_add(&stats[S_EXAMINED], examined);
-        wave_add(&stats[S_NEXT_EDGES], ndeg);
+        t_found += found;
+        t_examined += examined;
+        t_next += ndeg;
     }
+    const int words[3] = {S_FOUND, S_EXAMINED, S_NEXT_EDGES};
+    const unsigned long long vals[3] = {t_found, t_examined, t_next};
+    block_flush<3>(stats, words, vals);
+}
+
+// Heavy pass: one wave per item (a vertex of degree >= heavy_min, or one
+// <= 4096-edge chunk of a longer list). The wave checks 64 neighbours per round
+// and stops at the first round with a frontier hit (ballot). The first chunk
+// to set the vertex's bit in `next` (atomicOr) records it.
+__device__ __forceinline__ void ph_bu_heavy(const BfsDev& d, const uint64_t* front, uint64_t* next, uint32_t level,
+                                            unsigned long long* stats, uint32_t block, uint32_t grid) {
+    const uint32_t lane = lane_id();
+    const uint32_t wpb = blockDim.x / kWave, wid = threadIdx.x / kWave;
+    unsigned long long t_found = 0, t_examined = 0, t_next = 0;
+    for (uint32_t it = block * wpb + wid; it < d.n_heavy; it += grid * wpb) {
+        const uint4 item = d.heavy[it];
+        const uint32_t v = item.x, x0 = item.y, x1 = item.z;
+        const uint64_t bit = 1ull << (v & 63);
+        if (d.visited[v >> 6] & bit) continue;  // wave-uniform
+        uint32_t examined = 0;
+        bool hit_any = false;
+        for (uint32_t base = x0; base < x1; base += kWave) {
+            const uint32_t x = base + lane;
+            bool hit = false;
+            if (x < x1) {
+                const uint32_t u = d.ci[x];
+                hit = (front[u >> 6] >> (u & 63)) & 1ull;
+            }
+            const uint64_t m = __ballot(hit);
+            if (m) {
+                examined += (uint32_t)__ffsll((long long)m);
+                hit_any = true;
+                break;
+            }
+            examined += min(kWave, (int)(x1 - base));
+        }
+        if (lane == 0) {
+            if (hit_any) {
+                const uint64_t old = atomicOr((unsigned long long*)&next[v >> 6], (unsigned long long)bit);
+                if (!(old & bit)) {
+                    d.dist[v] = level + 1;
+                    atomicOr((unsigned long long*)&d.visited[v >> 6], (unsigned long long)bit);
+                    t_found += 1;
+                    t_next += item.w;
+                }
+            }
+            t_examined += examined;
+        }
+    }
+    const int words[3] = {S_FOUND, S_EXAMINED, S_NEXT_EDGES};
+    const unsigned long long vals[3] = {t_found, t_examined, t_next};
+    block_flush<3>(stats, words, vals);
 }
 
 // ------------------------------------------------------------ conversions

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <memory>
 #include <numeric>
 
 #include "graph.hpp"
@@ -29,81 +30,20 @@
 namespace ggpu {
 
 constexpr uint32_t kSecBlocks = 2048;             // block cap per section (grid-stride beyond)
-constexpr uint32_t kPartialCap = 3 * kSecBlocks;  // FP64 partials per quantity
+constexpr uint32_t kStreamBlocks = 8192;          // block cap of a stream-layout sweep
+constexpr uint32_t kPartialCap = 1u << 17;        // FP64 partial slots per quantity (all launches of a sweep)
 
 // ------------------------------------------------------------------ kernels
 __global__ void k_pr_sweep(PrSweep P) { pr_sweep_body(P); }
 
-// Shared tail: block partials, last block reduces into acc_out (overwrite).
-__device__ __forceinline__ void block_finish(PrPart part, double* partial, uint32_t* ticket, PrAcc* acc_out) {
-    __shared__ double red[4][16];
-    __shared__ bool last;
-    const uint32_t lane = lane_id(), w = threadIdx.x / kWave, nwv = blockDim.x / kWave;
-    double a = wave_sum(part.dang), b = wave_sum(part.delta), c = wave_sum(part.sum), m = wave_max(part.maxrel);
-    if (lane == 0) {
-        red[0][w] = a;
-        red[1][w] = b;
-        red[2][w] = c;
-        red[3][w] = m;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
-        for (uint32_t j = 0; j < nwv; ++j) {
-            t0 += red[0][j];
-            t1 += red[1][j];
-            t2 += red[2][j];
-            t3 = red[3][j] > t3 ? red[3][j] : t3;
-        }
-        partial[blockIdx.x] = t0;
-        partial[kPartialCap + blockIdx.x] = t1;
-        partial[2 * kPartialCap + blockIdx.x] = t2;
-        partial[3 * kPartialCap + blockIdx.x] = t3;
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
-    for (uint32_t j = threadIdx.x; j < gridDim.x; j += blockDim.x) {
-        t0 += partial[j];
-        t1 += partial[kPartialCap + j];
-        t2 += partial[2 * kPartialCap + j];
-        const double q = partial[3 * kPartialCap + j];
-        t3 = q > t3 ? q : t3;
-    }
-    t0 = wave_sum(t0);
-    t1 = wave_sum(t1);
-    t2 = wave_sum(t2);
-    t3 = wave_max(t3);
-    __syncthreads();
-    if (lane == 0) {
-        red[0][w] = t0;
-        red[1][w] = t1;
-        red[2][w] = t2;
-        red[3][w] = t3;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double T0 = 0, T1 = 0, T2 = 0, T3 = 0;
-        for (uint32_t j = 0; j < nwv; ++j) {
-            T0 += red[0][j];
-            T1 += red[1][j];
-            T2 += red[2][j];
-            T3 = red[3][j] > T3 ? red[3][j] : T3;
-        }
-        acc_out->dangling = T0;
-        acc_out->delta = T1;
-        acc_out->sum = T2;
-        acc_out->maxrel = T3;
-        *ticket = 0;
-    }
+__global__ void k_pr_longfix(PrSweep P) { pr_longfix_body(P); }
+__global__ void k_pr_reduce(const double* partial, uint32_t cap, uint32_t count, PrAcc* out) {
+    pr_reduce_body(partial, cap, count, out);
 }
 
-// r = 1/n, contrib = r/outdeg; acc_out = {dangling, 0, sum, 0}.
+// r = 1/n, contrib = r/outdeg; block partials of {dangling, 0, sum, 0}.
 __global__ void k_pr_init(float* r, float* contrib, const uint32_t* outdeg, uint32_t n, double* partial,
-                          uint32_t* ticket, PrAcc* acc_out) {
+                          uint32_t cap) {
     PrPart part;
     const float r0 = 1.0f / (float)n;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
@@ -113,13 +53,13 @@ __global__ void k_pr_init(float* r, float* contrib, const uint32_t* outdeg, uint
         if (!od) part.dang += (double)r0;
         part.sum += (double)r0;
     }
-    block_finish(part, partial, ticket, acc_out);
+    write_block_partials(part, partial, cap, blockIdx.x);
 }
 
 // After an in-place sweep: r_new *= 1/sum (sum from acc_sweep), L1 change
 // against the previous normalized ranks, new contributions and dangling mass.
 __global__ void k_pr_finish(float* r_new, const float* r_old, float* contrib, const uint32_t* outdeg, uint32_t n,
-                            const PrAcc* acc_sweep, double* partial, uint32_t* ticket, PrAcc* acc_out) {
+                            const PrAcc* acc_sweep, double* partial, uint32_t cap) {
     PrPart part;
     const float scale = (float)(1.0 / acc_sweep->sum);
     const float inv_n = 1.0f / (float)n;
@@ -136,34 +76,58 @@ __global__ void k_pr_finish(float* r_new, const float* r_old, float* contrib, co
         const double rel = (double)dl / (double)fmaxf(old, inv_n);
         part.maxrel = rel > part.maxrel ? rel : part.maxrel;
     }
-    block_finish(part, partial, ticket, acc_out);
+    write_block_partials(part, partial, cap, blockIdx.x);
 }
 
 // ------------------------------------------------------------------ engine
-struct PrEngine {
-    ggpu_ctx* ctx;
-    uint32_t n;
-    uint64_t m;
-    Csr in_host;
-    std::vector<uint32_t> od_host;
-    DBuf<uint32_t> rp, ci, od;
-    DBuf<float> r, contrib;  // 2n each
-    DBuf<double> partial;
-    DBuf<uint32_t> ticket;
-    DBuf<PrAcc> acc;         // [0],[1] Jacobi ping-pong / GS state, [2] GS sweep totals
-    // adaptive bins: [small | medium | large], vertex ids ascending within a bin
+// Row-block items of the stream layout over rows [r_begin, r_end).
+struct StreamLayout {
+    uint32_t r_begin = 0, r_end = 0;
+    DBuf<uint2> sitems, lrows;
+    DBuf<uint4> litems;
+    DBuf<float2> lpart;
+    uint32_t ns = 0, nl = 0, nlr = 0;
+    bool multi = false;  // some long row has more than one chunk
+};
+
+// One vertex numbering of the in-CSR with its device copy and work layouts.
+// Spaces: the caller's ids; descending out-degree (hot contributions packed
+// into few cache lines); the Gauss-Seidel colour order (colour sets are
+// contiguous id ranges); the Gauss-Seidel natural order with deferred pair
+// partners moved to the end. id_of[caller id] = id in the space.
+enum SpaceId { SP_GIVEN = 0, SP_DEGREE = 1, SP_COLOR = 2, SP_NATURAL = 3, SP_COUNT = 4 };
+
+struct Space {
+    bool built = false;
+    Csr in;
+    std::vector<uint32_t> od;
+    std::vector<uint32_t> id_of;  // empty: identity
+    std::vector<std::pair<uint32_t, uint32_t>> ranges;  // GS: consecutive update ranges
+    DBuf<uint32_t> rp, ci, odd;
+    std::vector<std::unique_ptr<StreamLayout>> stream;  // one per range
     DBuf<uint32_t> bins;
     uint32_t bin_count[3] = {0, 0, 0};
     uint32_t bin_small = 0, bin_large = 0;
     bool bins_ready = false;
-    // Gauss-Seidel order
-    GsOrder gs;
-    bool gs_ready = false;
+};
+
+struct PrEngine {
+    ggpu_ctx* ctx;
+    uint32_t n;
+    uint64_t m;
+    Space sp[SP_COUNT];
+    DBuf<float> r, contrib;  // 2n each
+    DBuf<double> partial;
+    uint32_t npart = 0;      // partial slots written since the last reduction
+    DBuf<PrAcc> acc;         // [0],[1] Jacobi ping-pong / in-place state, [2] in-place sweep totals
+    // Gauss-Seidel decision (made on the caller's numbering)
+    bool gs_ready = false, gs_colored = false;
     uint32_t gs_pair_k = 0;
-    DBuf<uint32_t> gs_rp, gs_ci, gs_od, gs_lists;
+    int natural_base = -1;
+    uint64_t gs_deferred = 0;
     // result
-    int result_slot = 0;      // which half of r holds the last ranks
-    bool result_relabeled = false;
+    int result_slot = 0;
+    int result_space = SP_GIVEN;
     bool have_result = false;
     std::vector<hipEvent_t> ev;
     // personalized PageRank block (ppr.hip)
@@ -173,22 +137,16 @@ struct PrEngine {
         : ctx(c), n((uint32_t)n_), m(m_) {
         if (n_ >= GGPU_INF) throw ArgError("graph too large for 32-bit vertex ids");
         if (m_ >= (1ull << 32)) throw ArgError("graph has 2^32 or more arcs");
-        in_host = from_arrays(n_, m_, hrp, hci, GGPU_FLAG_SORTED);
-        od_host.assign(hod, hod + n_);
         GGPU_HIP_TRY(hipSetDevice(ctx->device));
-        rp.alloc(n + 1);
-        ci.alloc(m ? m : 1);
-        od.alloc(n ? n : 1);
+        Space& g = sp[SP_GIVEN];
+        g.in = from_arrays(n_, m_, hrp, hci, GGPU_FLAG_SORTED);
+        g.od.assign(hod, hod + n_);
+        upload(g);
         r.alloc(2ull * (n ? n : 1));
         contrib.alloc(2ull * (n ? n : 1));
         partial.alloc(4ull * kPartialCap);
-        ticket.alloc(1);
         acc.alloc(4);
         hipStream_t s = ctx->stream;
-        GGPU_HIP_TRY(hipMemcpyAsync(rp.p, hrp, 4ull * (n + 1), hipMemcpyHostToDevice, s));
-        if (m) GGPU_HIP_TRY(hipMemcpyAsync(ci.p, hci, 4ull * m, hipMemcpyHostToDevice, s));
-        if (n) GGPU_HIP_TRY(hipMemcpyAsync(od.p, hod, 4ull * n, hipMemcpyHostToDevice, s));
-        GGPU_HIP_TRY(hipMemsetAsync(ticket.p, 0, 4, s));
         GGPU_HIP_TRY(hipMemsetAsync(acc.p, 0, 4 * sizeof(PrAcc), s));
         GGPU_HIP_TRY(hipStreamSynchronize(s));
     }
@@ -204,45 +162,160 @@ struct PrEngine {
         return ev[i];
     }
 
-    void prepare_adaptive(uint32_t small_max, uint32_t large_min) {
-        if (bins_ready && bin_small == small_max && bin_large == large_min) return;
+    void upload(Space& S) {
+        S.rp.alloc(n + 1);
+        S.ci.alloc(S.in.m ? S.in.m : 1);
+        S.odd.alloc(n ? n : 1);
+        GGPU_HIP_TRY(hipMemcpy(S.rp.p, S.in.rp.data(), 4ull * (n + 1), hipMemcpyHostToDevice));
+        if (S.in.m) GGPU_HIP_TRY(hipMemcpy(S.ci.p, S.in.ci.data(), 4ull * S.in.m, hipMemcpyHostToDevice));
+        if (n) GGPU_HIP_TRY(hipMemcpy(S.odd.p, S.od.data(), 4ull * n, hipMemcpyHostToDevice));
+        if (S.ranges.empty()) S.ranges.push_back({0u, n});
+        S.built = true;
+    }
+
+    // Relabels `base` by perm (perm[base id] = new id) into space `dst`.
+    void derive(int dst, const Space& base, const std::vector<uint32_t>& perm) {
+        Space& S = sp[dst];
+        S.in = permute(base.in, perm);
+        S.od.assign(n, 0);
+        for (uint32_t v = 0; v < n; ++v) S.od[perm[v]] = base.od[v];
+        S.id_of.resize(n);
+        for (uint32_t v = 0; v < n; ++v) S.id_of[v] = perm[base.id_of.empty() ? v : base.id_of[v]];
+        S.stream.clear();
+        S.bins_ready = false;
+    }
+
+    Space& space(int id) {
+        Space& S = sp[id];
+        if (S.built) return S;
+        if (id == SP_DEGREE) {
+            // descending out-degree, ties by id: the most gathered contributions first
+            const Space& g = sp[SP_GIVEN];
+            std::vector<uint32_t> order(n);
+            std::iota(order.begin(), order.end(), 0u);
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return g.od[a] > g.od[b]; });
+            std::vector<uint32_t> perm(n);
+            for (uint32_t i = 0; i < n; ++i) perm[order[i]] = i;
+            derive(SP_DEGREE, g, perm);
+            upload(S);
+            return S;
+        }
+        throw ArgError("space not prepared");
+    }
+
+    void build_stream(Space& S) {
+        if (S.stream.size() == S.ranges.size()) return;
+        S.stream.clear();
+        for (auto [ra, rb] : S.ranges) {
+            auto L = std::make_unique<StreamLayout>();
+            L->r_begin = ra;
+            L->r_end = rb;
+            std::vector<uint2> sitems, lrows;
+            std::vector<uint4> litems;
+            uint32_t cur0 = ra, rows = 0, nnz = 0;
+            auto close = [&](uint32_t end) {
+                if (rows) sitems.push_back(make_uint2(cur0, end));
+                rows = 0;
+                nnz = 0;
+            };
+            for (uint32_t v = ra; v < rb; ++v) {
+                const uint32_t dg = S.in.deg(v);
+                if (dg > kStreamNnz) {
+                    close(v);
+                    const uint32_t first = (uint32_t)litems.size();
+                    for (uint32_t x = S.in.rp[v]; x < S.in.rp[v + 1]; x += kLongChunk)
+                        litems.push_back(
+                            make_uint4(v, x, std::min(x + kLongChunk, S.in.rp[v + 1]), (uint32_t)lrows.size()));
+                    lrows.push_back(make_uint2(first, (uint32_t)litems.size() - first));
+                    continue;
+                }
+                if (rows && (nnz + dg > kStreamNnz || rows == kStreamRows)) close(v);
+                if (!rows) cur0 = v;
+                ++rows;
+                nnz += dg;
+            }
+            close(rb);
+            L->ns = (uint32_t)sitems.size();
+            L->nl = (uint32_t)litems.size();
+            L->nlr = (uint32_t)lrows.size();
+            L->sitems.alloc(L->ns + 1);
+            L->litems.alloc(L->nl + 1);
+            L->lrows.alloc(L->nlr + 1);
+            L->lpart.alloc(L->nl + 1);
+            for (const uint2& lr : lrows) L->multi |= lr.y > 1;
+            if (L->ns) GGPU_HIP_TRY(hipMemcpy(L->sitems.p, sitems.data(), L->ns * sizeof(uint2), hipMemcpyHostToDevice));
+            if (L->nl) GGPU_HIP_TRY(hipMemcpy(L->litems.p, litems.data(), L->nl * sizeof(uint4), hipMemcpyHostToDevice));
+            if (L->nlr) GGPU_HIP_TRY(hipMemcpy(L->lrows.p, lrows.data(), L->nlr * sizeof(uint2), hipMemcpyHostToDevice));
+            S.stream.push_back(std::move(L));
+        }
+    }
+
+    void build_bins(Space& S, uint32_t small_max, uint32_t large_min) {
+        if (S.bins_ready && S.bin_small == small_max && S.bin_large == large_min) return;
         std::vector<uint32_t> lists[3];
         for (uint32_t v = 0; v < n; ++v) {
-            const uint32_t d = in_host.deg(v);
+            const uint32_t d = S.in.deg(v);
             lists[d < small_max ? 0 : (d < large_min ? 1 : 2)].push_back(v);
         }
         std::vector<uint32_t> all;
         all.reserve(n);
         for (int b = 0; b < 3; ++b) {
-            bin_count[b] = (uint32_t)lists[b].size();
+            S.bin_count[b] = (uint32_t)lists[b].size();
             all.insert(all.end(), lists[b].begin(), lists[b].end());
         }
-        bins.alloc(n ? n : 1);
-        if (n) GGPU_HIP_TRY(hipMemcpy(bins.p, all.data(), 4ull * n, hipMemcpyHostToDevice));
-        bin_small = small_max;
-        bin_large = large_min;
-        bins_ready = true;
+        S.bins.alloc(n ? n : 1);
+        if (n) GGPU_HIP_TRY(hipMemcpy(S.bins.p, all.data(), 4ull * n, hipMemcpyHostToDevice));
+        S.bin_small = small_max;
+        S.bin_large = large_min;
+        S.bins_ready = true;
     }
 
-    void prepare_gs(uint32_t pair_k) {
-        if (gs_ready && gs_pair_k == pair_k) return;
-        gs = gs_order(in_host, od_host.data(), pair_k);
-        gs_pair_k = pair_k;
-        if (gs.colored) {
-            const Csr& R = gs.relabeled_in;
-            gs_rp.alloc(n + 1);
-            gs_ci.alloc(R.m ? R.m : 1);
-            gs_od.alloc(n ? n : 1);
-            GGPU_HIP_TRY(hipMemcpy(gs_rp.p, R.rp.data(), 4ull * (n + 1), hipMemcpyHostToDevice));
-            if (R.m) GGPU_HIP_TRY(hipMemcpy(gs_ci.p, R.ci.data(), 4ull * R.m, hipMemcpyHostToDevice));
-            if (n) GGPU_HIP_TRY(hipMemcpy(gs_od.p, gs.relabeled_outdeg.data(), 4ull * n, hipMemcpyHostToDevice));
-        } else {
-            std::vector<uint32_t> both(gs.main_list);
-            both.insert(both.end(), gs.deferred_list.begin(), gs.deferred_list.end());
-            gs_lists.alloc(n ? n : 1);
-            if (n) GGPU_HIP_TRY(hipMemcpy(gs_lists.p, both.data(), 4ull * n, hipMemcpyHostToDevice));
+    void prepare_adaptive(uint32_t small_max, uint32_t large_min) { build_bins(sp[SP_GIVEN], small_max, large_min); }
+
+    // Gauss-Seidel order (pagerank_gpu.mm:88-192): colour sets if they cover
+    // >= 95% of the vertices, else natural order with pair deferral. The
+    // natural order is built on `base` (the caller's or the degree order).
+    void prepare_gs(uint32_t pair_k, int base_id = SP_GIVEN) {
+        if (gs_ready && gs_pair_k == pair_k && (gs_colored || natural_base == base_id)) return;
+        Space& g = sp[SP_GIVEN];
+        if (!gs_ready || gs_pair_k != pair_k) {
+            GsOrder o = gs_order(g.in, g.od.data(), pair_k);
+            gs_colored = o.colored;
+            gs_pair_k = pair_k;
+            gs_ready = true;
+            if (o.colored) {
+                Space& C = sp[SP_COLOR];
+                C.in = std::move(o.relabeled_in);
+                C.od = std::move(o.relabeled_outdeg);
+                C.id_of = std::move(o.new_id);
+                C.ranges.clear();
+                for (size_t i = 0; i + 1 < o.set_begin.size(); ++i)
+                    C.ranges.push_back({o.set_begin[i], o.set_begin[i + 1]});
+                C.stream.clear();
+                C.bins_ready = false;
+                upload(C);
+                gs_deferred = 0;
+                natural_base = -1;
+                return;
+            }
+            natural_base = -1;
         }
-        gs_ready = true;
+        if (gs_colored) return;
+        Space& B = space(base_id);
+        GsOrder o = gs_order(B.in, B.od.data(), pair_k, false);
+        std::vector<uint32_t> perm(n);
+        uint32_t k = 0;
+        for (uint32_t v : o.main_list) perm[v] = k++;
+        const uint32_t split = k;
+        for (uint32_t v : o.deferred_list) perm[v] = k++;
+        derive(SP_NATURAL, B, perm);
+        Space& N = sp[SP_NATURAL];
+        N.ranges.clear();
+        N.ranges.push_back({0u, split});
+        if (split < n) N.ranges.push_back({split, n});
+        upload(N);
+        gs_deferred = o.deferred_list.size();
+        natural_base = base_id;
     }
 
     static uint32_t blocks(uint64_t items, uint32_t per_block) {
@@ -250,7 +323,6 @@ struct PrEngine {
         return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, kSecBlocks));
     }
 
-    // Builds the sections of one launch over [list or range]; returns the grid.
     uint32_t add_section(PrSweep& P, const uint32_t* list, uint32_t first, uint32_t count, uint32_t mode,
                          uint32_t lanes, uint32_t tg, uint32_t grid) {
         if (count == 0) return grid;
@@ -269,13 +341,27 @@ struct PrEngine {
         return grid + nb;
     }
 
-    PrSweep base_sweep(bool relabeled, float damping) {
+    // Stream-layout launch geometry: long-chunk blocks first, then row blocks.
+    uint32_t use_stream(PrSweep& P, StreamLayout& L) {
+        PrStreamLayout& Y = P.stream;
+        Y.sitems = L.sitems.p;
+        Y.n_sitems = L.ns;
+        Y.litems = L.litems.p;
+        Y.n_litems = L.nl;
+        Y.lrows = L.lrows.p;
+        Y.lpart = L.lpart.p;
+        Y.n_lrows = L.nlr;
+        P.use_stream = 1;
+        P.stream_multi = L.multi;
+        return std::max<uint32_t>(1, std::min<uint32_t>(L.ns + L.nl, kStreamBlocks));
+    }
+
+    PrSweep base_sweep(const Space& S, float damping) {
         PrSweep P{};
-        P.rp = relabeled ? gs_rp.p : rp.p;
-        P.ci = relabeled ? gs_ci.p : ci.p;
-        P.outdeg = relabeled ? gs_od.p : od.p;
+        P.rp = S.rp.p;
+        P.ci = S.ci.p;
+        P.outdeg = S.odd.p;
         P.partial = partial.p;
-        P.ticket = ticket.p;
         P.partial_cap = kPartialCap;
         P.n = n;
         P.damping = damping;
@@ -283,94 +369,96 @@ struct PrEngine {
     }
 
     void launch(PrSweep& P, uint32_t grid, uint32_t tg) {
-        if (P.nsec == 0) return;
+        if (P.nsec == 0 && !P.use_stream) return;
+        if (npart + grid > kPartialCap) throw ArgError("too many blocks in one sweep");
+        P.partial_base = npart;
+        npart += grid;
         k_pr_sweep<<<grid, tg, 0, ctx->stream>>>(P);
         GGPU_HIP_TRY(hipGetLastError());
+        if (P.use_stream && P.stream_multi) {
+            const uint32_t g = blocks(P.stream.n_lrows, 256 / kWave);
+            P.partial_base = npart;
+            npart += g;
+            k_pr_longfix<<<g, 256, 0, ctx->stream>>>(P);
+            GGPU_HIP_TRY(hipGetLastError());
+        }
     }
 
-    void init(const uint32_t* outdeg, PrAcc* acc_out) {
-        k_pr_init<<<blocks(n, 256), 256, 0, ctx->stream>>>(r.p, contrib.p, outdeg, n, partial.p, ticket.p, acc_out);
+    // Reduces the partial slots written since the last reduction into *out.
+    void reduce(PrAcc* out) {
+        k_pr_reduce<<<1, 1024, 0, ctx->stream>>>(partial.p, kPartialCap, npart, out);
         GGPU_HIP_TRY(hipGetLastError());
+        npart = 0;
+    }
+
+    // Sections of a Jacobi-variant sweep over all rows of S.
+    uint32_t jacobi_layout(PrSweep& P, Space& S, int variant, const ggpu_pr_opts& o) {
+        const uint32_t tg = o.tg;
+        uint32_t grid = 0;
+        if (variant == GGPU_PR_SCALAR) return add_section(P, nullptr, 0, n, SEC_LANE, 1, tg, grid);
+        if (variant == GGPU_PR_VECTOR) return add_section(P, nullptr, 0, n, SEC_GROUP, kWave, tg, grid);
+        if (o.layout == GGPU_PR_LAYOUT_STREAM) {
+            build_stream(S);
+            return use_stream(P, *S.stream[0]);
+        }
+        build_bins(S, o.small_max, o.large_min);
+        grid = add_section(P, S.bins.p + S.bin_count[0] + S.bin_count[1], 0, S.bin_count[2], SEC_BLOCK, 1, tg, grid);
+        grid = add_section(P, S.bins.p + S.bin_count[0], 0, S.bin_count[1], SEC_GROUP, o.medium_lanes, tg, grid);
+        return add_section(P, S.bins.p, 0, S.bin_count[0], SEC_LANE, 1, tg, grid);
     }
 
     // One Jacobi sweep k (contrib[k&1] -> contrib[(k+1)&1], r[k&1] -> r[(k+1)&1]).
-    void jacobi_sweep(int variant, const ggpu_pr_opts& o, int k, bool want_delta) {
+    void jacobi_sweep(Space& S, int variant, const ggpu_pr_opts& o, int k, bool want_delta) {
         const int c = k & 1, nx = c ^ 1;
-        PrSweep P = base_sweep(false, o.damping);
+        PrSweep P = base_sweep(S, o.damping);
         P.contrib_in = contrib.p + (size_t)c * n;
         P.contrib_out = contrib.p + (size_t)nx * n;
         P.r_old = want_delta ? r.p + (size_t)c * n : nullptr;
         P.r_new = r.p + (size_t)nx * n;
         P.acc_in = acc.p + c;
-        P.acc_out = acc.p + nx;
-        P.first = 1;
-        uint32_t grid = 0;
-        const uint32_t tg = o.tg;
-        if (variant == GGPU_PR_SCALAR) grid = add_section(P, nullptr, 0, n, SEC_LANE, 1, tg, grid);
-        else if (variant == GGPU_PR_VECTOR) grid = add_section(P, nullptr, 0, n, SEC_GROUP, kWave, tg, grid);
-        else {
-            grid = add_section(P, bins.p + bin_count[0] + bin_count[1], 0, bin_count[2], SEC_BLOCK, 1, tg, grid);
-            grid = add_section(P, bins.p + bin_count[0], 0, bin_count[1], SEC_GROUP, o.medium_lanes, tg, grid);
-            grid = add_section(P, bins.p, 0, bin_count[0], SEC_LANE, 1, tg, grid);
-        }
-        launch(P, grid, tg);
+        const uint32_t grid = jacobi_layout(P, S, variant, o);
+        launch(P, grid, o.tg);
+        reduce(acc.p + nx);
     }
 
     // One in-place sweep (gpu_pull_gs, or a Jacobi variant with in_place) + finish.
-    void inplace_sweep(int variant, const ggpu_pr_opts& o, int k) {
-        const bool relabeled = variant == GGPU_PR_GS && gs.colored;
+    void inplace_sweep(Space& S, int variant, const ggpu_pr_opts& o, int k) {
         const int c = k & 1, nx = c ^ 1;
         float* rin = r.p + (size_t)c * n;
         float* rout = r.p + (size_t)nx * n;
-        auto fresh = [&](uint32_t first) {
-            PrSweep P = base_sweep(relabeled, o.damping);
+        auto fresh = [&](uint32_t) {
+            PrSweep P = base_sweep(S, o.damping);
             P.contrib_in = contrib.p;
             P.contrib_out = contrib.p;
             P.r_old = nullptr;
             P.r_new = rout;
             P.acc_in = acc.p + 0;
-            P.acc_out = acc.p + 2;
-            P.first = first;
             return P;
         };
         const uint32_t tg = o.tg;
-        uint32_t launches = 0;
-        if (variant == GGPU_PR_GS && gs.colored) {
-            // one lane-per-vertex launch per colour set, in order
-            for (size_t s = 0; s + 1 < gs.set_begin.size(); ++s) {
-                PrSweep P = fresh(launches == 0);
-                const uint32_t a = gs.set_begin[s], b = gs.set_begin[s + 1];
-                const uint32_t grid = add_section(P, nullptr, a, b - a, SEC_LANE, 1, tg, 0);
+        if (variant == GGPU_PR_GS) {
+            // ranges in order: colour sets, or natural order then deferred partners
+            if (o.layout == GGPU_PR_LAYOUT_STREAM) build_stream(S);
+            for (size_t i = 0; i < S.ranges.size(); ++i) {
+                PrSweep P = fresh(i == 0);
+                const auto [a, b] = S.ranges[i];
+                uint32_t grid;
+                if (o.layout == GGPU_PR_LAYOUT_STREAM) grid = use_stream(P, *S.stream[i]);
+                else if (gs_colored) grid = add_section(P, nullptr, a, b - a, SEC_LANE, 1, tg, 0);
+                else grid = add_section(P, nullptr, a, b - a, SEC_GROUP, kWave, tg, 0);
                 launch(P, grid, tg);
-                ++launches;
-            }
-        } else if (variant == GGPU_PR_GS) {
-            // natural order, wave per vertex; deferred pair partners in a second launch
-            const uint32_t na = (uint32_t)gs.main_list.size(), nb = (uint32_t)gs.deferred_list.size();
-            PrSweep P = fresh(1);
-            uint32_t grid = add_section(P, gs_lists.p, 0, na, SEC_GROUP, kWave, tg, 0);
-            launch(P, grid, tg);
-            ++launches;
-            if (nb) {
-                PrSweep Q = fresh(0);
-                grid = add_section(Q, gs_lists.p + na, 0, nb, SEC_GROUP, kWave, tg, 0);
-                launch(Q, grid, tg);
             }
         } else {
             PrSweep P = fresh(1);
-            uint32_t grid = 0;
-            if (variant == GGPU_PR_SCALAR) grid = add_section(P, nullptr, 0, n, SEC_LANE, 1, tg, grid);
-            else if (variant == GGPU_PR_VECTOR) grid = add_section(P, nullptr, 0, n, SEC_GROUP, kWave, tg, grid);
-            else {
-                grid = add_section(P, bins.p + bin_count[0] + bin_count[1], 0, bin_count[2], SEC_BLOCK, 1, tg, grid);
-                grid = add_section(P, bins.p + bin_count[0], 0, bin_count[1], SEC_GROUP, o.medium_lanes, tg, grid);
-                grid = add_section(P, bins.p, 0, bin_count[0], SEC_LANE, 1, tg, grid);
-            }
+            const uint32_t grid = jacobi_layout(P, S, variant, o);
             launch(P, grid, tg);
         }
-        k_pr_finish<<<blocks(n, 256), 256, 0, ctx->stream>>>(rout, rin, contrib.p, relabeled ? gs_od.p : od.p, n,
-                                                              acc.p + 2, partial.p, ticket.p, acc.p + 0);
+        reduce(acc.p + 2);  // sweep totals (sum for the renormalization)
+        const uint32_t g = blocks(n, 256);
+        k_pr_finish<<<g, 256, 0, ctx->stream>>>(rout, rin, contrib.p, S.odd.p, n, acc.p + 2, partial.p, kPartialCap);
         GGPU_HIP_TRY(hipGetLastError());
+        npart = g;
+        reduce(acc.p + 0);
     }
 
     ggpu_pr_result run(int variant, const ggpu_pr_opts& o) {
@@ -379,20 +467,31 @@ struct PrEngine {
         const uint32_t L = o.medium_lanes;
         if (L == 0 || L > 64 || (L & (L - 1))) throw ArgError("medium_lanes must be a power of two in [1, 64]");
         if (o.iters < 0) throw ArgError("iters must be >= 0");
-        if (variant == GGPU_PR_ADAPTIVE) prepare_adaptive(o.small_max, o.large_min);
-        if (variant == GGPU_PR_GS && !gs_ready) prepare_gs(2);
+        if (o.layout > GGPU_PR_LAYOUT_BINS) throw ArgError("unknown PageRank layout");
+        if (o.order > GGPU_PR_ORDER_DEGREE) throw ArgError("unknown PageRank vertex order");
+        const int base_id = o.order == GGPU_PR_ORDER_DEGREE ? SP_DEGREE : SP_GIVEN;
+        int sid = base_id;
+        if (variant == GGPU_PR_GS) {
+            prepare_gs(gs_ready ? gs_pair_k : 2, base_id);
+            sid = gs_colored ? SP_COLOR : SP_NATURAL;
+        }
+        Space& S = space(sid);
         const bool inplace = variant == GGPU_PR_GS || o.in_place;
-        if (inplace && variant == GGPU_PR_ADAPTIVE) prepare_adaptive(o.small_max, o.large_min);
         ggpu_pr_result res{};
         res.last_delta = -1;
         res.last_max_rel = -1;
         have_result = false;
-        result_relabeled = variant == GGPU_PR_GS && gs.colored;
+        result_space = sid;
         if (n == 0) return res;
         hipStream_t s = ctx->stream;
         const bool tol_mode = o.tol > 0;
         GGPU_HIP_TRY(hipEventRecord(event(0), s));
-        init(result_relabeled ? gs_od.p : od.p, acc.p + 0);
+        npart = 0;
+        const uint32_t gi = blocks(n, 256);
+        k_pr_init<<<gi, 256, 0, s>>>(r.p, contrib.p, S.odd.p, n, partial.p, kPartialCap);
+        GGPU_HIP_TRY(hipGetLastError());
+        npart = gi;
+        reduce(acc.p + 0);
         auto* pin = (PrAcc*)ctx_pinned(ctx, 4096);
         double best = 1e300;
         int since_best = 0;
@@ -400,17 +499,16 @@ struct PrEngine {
         for (; k < o.iters; ++k) {
             const PrAcc* out;
             if (inplace) {
-                inplace_sweep(variant, o, k);
+                inplace_sweep(S, variant, o, k);
                 out = acc.p + 0;
             } else {
-                jacobi_sweep(variant, o, k, tol_mode);
+                jacobi_sweep(S, variant, o, k, tol_mode);
                 out = acc.p + ((k + 1) & 1);
             }
             if (!tol_mode) continue;
             // tol mode: one synchronization per sweep to read the FP64 change
             GGPU_HIP_TRY(hipMemcpyAsync(pin, out, sizeof(PrAcc), hipMemcpyDeviceToHost, s));
             GGPU_HIP_TRY(hipStreamSynchronize(s));
-            res.iters = k + 1;
             res.last_delta = pin->delta;
             res.last_max_rel = pin->maxrel;
             res.dangling = pin->dangling;
@@ -435,8 +533,7 @@ struct PrEngine {
         GGPU_HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
         res.gpu_ms = ms;
         res.iters = k;
-        // ranks live in r[k & 1]: Jacobi writes r[(k+1)&1] in sweep k; in place too
-        result_slot = k & 1;
+        result_slot = k & 1;  // sweep k writes r[(k+1)&1]
         if (!tol_mode) {
             const PrAcc* out = inplace ? acc.p + 0 : acc.p + (k & 1);
             GGPU_HIP_TRY(hipMemcpy(pin, out, sizeof(PrAcc), hipMemcpyDeviceToHost));
@@ -450,43 +547,52 @@ struct PrEngine {
     void fetch_ranks(float* out) {
         if (!have_result) throw ArgError("no PageRank result yet: call ggpu_pr_run first", GGPU_ESTATE);
         const float* src = r.p + (size_t)result_slot * n;
-        if (!result_relabeled) {
+        const Space& S = sp[result_space];
+        if (S.id_of.empty()) {
             GGPU_HIP_TRY(hipMemcpy(out, src, 4ull * n, hipMemcpyDeviceToHost));
             return;
         }
         std::vector<float> tmp(n);
         GGPU_HIP_TRY(hipMemcpy(tmp.data(), src, 4ull * n, hipMemcpyDeviceToHost));
-        for (uint32_t v = 0; v < n; ++v) out[v] = tmp[gs.new_id[v]];
+        for (uint32_t v = 0; v < n; ++v) out[v] = tmp[S.id_of[v]];
     }
 
-    // Time `iters` Jacobi sweeps of gpu_pull_adaptive for each candidate.
+    // Times `iters` Jacobi sweeps of gpu_pull_adaptive per candidate layout /
+    // order / bin shape and keeps the fastest (ranks do not depend on it).
     void autotune(int iters, ggpu_pr_opts& o) {
-        const uint32_t smalls[] = {4, 8, 16, 32};
-        const uint32_t larges[] = {256, 1024, 4096};
-        const uint32_t lanes[] = {8, 16, 32, 64};
+        struct Cand { uint32_t layout, order, sm, lg, L; };
+        std::vector<Cand> cands = {{GGPU_PR_LAYOUT_STREAM, GGPU_PR_ORDER_GIVEN, 0, 0, 64},
+                                   {GGPU_PR_LAYOUT_STREAM, GGPU_PR_ORDER_DEGREE, 0, 0, 64}};
+        for (uint32_t order : {(uint32_t)GGPU_PR_ORDER_GIVEN, (uint32_t)GGPU_PR_ORDER_DEGREE})
+            for (uint32_t sm : {4u, 16u})
+                for (uint32_t lg : {256u, 4096u})
+                    for (uint32_t L : {16u, 64u}) cands.push_back({GGPU_PR_LAYOUT_BINS, order, sm, lg, L});
         float best = 1e30f;
         ggpu_pr_opts bo = o;
-        for (uint32_t sm : smalls)
-            for (uint32_t lg : larges)
-                for (uint32_t L : lanes) {
-                    if (L > 4 * sm * 2 && L > 16) continue;  // wide groups only pay off for long lists
-                    ggpu_pr_opts t = o;
-                    t.small_max = sm;
-                    t.large_min = lg;
-                    t.medium_lanes = L;
-                    t.iters = iters;
-                    t.tol = 0;
-                    t.in_place = 0;
-                    run(GGPU_PR_ADAPTIVE, t);  // warm-up (also builds the bins)
-                    ggpu_pr_result rr = run(GGPU_PR_ADAPTIVE, t);
-                    if (rr.gpu_ms < best) {
-                        best = (float)rr.gpu_ms;
-                        bo = t;
-                    }
-                }
+        for (const Cand& c : cands) {
+            ggpu_pr_opts t = o;
+            t.layout = c.layout;
+            t.order = c.order;
+            if (c.layout == GGPU_PR_LAYOUT_BINS) {
+                t.small_max = c.sm;
+                t.large_min = c.lg;
+                t.medium_lanes = c.L;
+            }
+            t.iters = iters;
+            t.tol = 0;
+            t.in_place = 0;
+            run(GGPU_PR_ADAPTIVE, t);  // warm-up (also builds the layout)
+            const ggpu_pr_result rr = run(GGPU_PR_ADAPTIVE, t);
+            if (rr.gpu_ms < best) {
+                best = (float)rr.gpu_ms;
+                bo = t;
+            }
+        }
         o.small_max = bo.small_max;
         o.large_min = bo.large_min;
         o.medium_lanes = bo.medium_lanes;
+        o.layout = bo.layout;
+        o.order = bo.order;
     }
 };
 
@@ -500,7 +606,10 @@ struct ggpu_pr {
 };
 
 namespace ggpu {
-PprView ppr_view(PrEngine& e) { return PprView{e.ctx, e.n, e.rp.p, e.ci.p, e.od.p, e.od_host.data()}; }
+PprView ppr_view(PrEngine& e) {
+    const Space& g = e.sp[SP_GIVEN];
+    return PprView{e.ctx, e.n, g.rp.p, g.ci.p, g.odd.p, g.od.data()};
+}
 PprState& ppr_state(PrEngine& e) { return e.ppr; }
 }  // namespace ggpu
 
@@ -518,6 +627,8 @@ void ggpu_pr_opts_default(ggpu_pr_opts* o) {
     o->large_min = 256;
     o->in_place = 0;
     o->medium_lanes = 64;
+    o->layout = GGPU_PR_LAYOUT_STREAM;
+    o->order = GGPU_PR_ORDER_GIVEN;
 }
 
 int ggpu_pr_parse_variant(const char* name, int* variant) {
@@ -563,9 +674,9 @@ int ggpu_pr_prepare_gs(ggpu_pr* p, uint32_t pair_max_outdeg) {
 int ggpu_pr_gs_info(const ggpu_pr* p, int* colored, uint64_t* sets, uint64_t* deferred) {
     if (!p) return GGPU_EINVAL;
     const PrEngine& e = *p->eng;
-    if (colored) *colored = e.gs_ready && e.gs.colored;
-    if (sets) *sets = e.gs.colored && !e.gs.set_begin.empty() ? e.gs.set_begin.size() - 1 : 0;
-    if (deferred) *deferred = e.gs.deferred_list.size();
+    if (colored) *colored = e.gs_ready && e.gs_colored;
+    if (sets) *sets = e.gs_ready && e.gs_colored ? e.sp[SP_COLOR].ranges.size() : 0;
+    if (deferred) *deferred = e.gs_deferred;
     return GGPU_OK;
 }
 

@@ -18,6 +18,10 @@
 namespace ggpu {
 
 constexpr int kMaxSections = 3;
+constexpr uint32_t kStreamNnz = 2048;
+constexpr uint32_t kStreamRows = 1024;
+constexpr uint32_t kLongChunk = 2048;
+
 enum SectionMode : uint32_t { SEC_LANE = 0, SEC_GROUP = 1, SEC_BLOCK = 2 };
 
 struct PrSection {
@@ -28,6 +32,16 @@ struct PrSection {
     uint32_t lanes;        // SEC_GROUP: lanes per vertex (power of two <= 64)
     uint32_t blk_begin;    // block range [blk_begin, blk_end) of this section
     uint32_t blk_end;
+};
+
+struct PrStreamLayout {
+    const uint2* sitems;  // {first row, end row}
+    uint32_t n_sitems;
+    const uint4* litems;  // {row, first edge, end edge, long-row index}
+    uint32_t n_litems;
+    const uint2* lrows;   // {first item, item count} per long row
+    float2* lpart;        // per long item (hi, lo)
+    uint32_t n_lrows;
 };
 
 // FP64 accumulators of one sweep.
@@ -47,15 +61,16 @@ struct PrSweep {
     const float* r_old;       // previous ranks (for delta / maxrel); may be null
     float* r_new;
     const PrAcc* acc_in;      // dangling of the state being pulled from
-    PrAcc* acc_out;           // this sweep's totals (set when first, else accumulated)
-    double* partial;          // 4 * partial_cap doubles
-    uint32_t* ticket;
+    double* partial;          // 4 * partial_cap doubles; this launch writes slots partial_base + block
     uint32_t partial_cap;
+    uint32_t partial_base;
     uint32_t n;
     float damping;
-    uint32_t first;           // first launch of the sweep: overwrite acc_out
     PrSection sec[kMaxSections];
     uint32_t nsec;
+    PrStreamLayout stream;    // used when use_stream != 0 (instead of the sections)
+    uint32_t use_stream;
+    uint32_t stream_multi;    // some long row spans several chunks (needs pr_longfix_body)
 };
 
 struct Kahan {
@@ -113,6 +128,147 @@ __device__ __forceinline__ float in_sum_lane(const PrSweep& P, uint32_t v) {
     return k.s - k.c;
 }
 
+// ------------------------------------------------------------ stream layout
+// gpu_pull_adaptive's default layout (the CSR-Adaptive scheme for SpMV):
+//  * row blocks: consecutive vertices whose in-lists total <= kStreamNnz
+//    entries. The block reads the col_idx span once, coalesced, and issues
+//    every contrib gather of the span at once into LDS; then G lanes per row
+//    (G = blockDim / rows, a power of two <= 64) sum that row's LDS slice with
+//    Kahan and a TwoSum combine.
+//  * long rows (> kStreamNnz entries): chunks of <= kLongChunk entries, one
+//    block each; chunk partials (hi, lo) are combined in chunk order by the
+//    last chunk to finish (deterministic).
+
+__device__ __forceinline__ void block_comp_reduce(float& hi, float& lo, float* s_hi, float* s_lo) {
+    const uint32_t lane = lane_id(), w = threadIdx.x / kWave, nwv = blockDim.x / kWave;
+    comp_reduce(hi, lo, kWave);
+    __syncthreads();
+    if (lane == 0) {
+        s_hi[w] = hi;
+        s_lo[w] = lo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float H = s_hi[0], L = s_lo[0];
+        for (uint32_t j = 1; j < nwv; ++j) {
+            float a, e;
+            two_sum(H, s_hi[j], a, e);
+            H = a;
+            L = L + s_lo[j] + e;
+        }
+        hi = H;
+        lo = L;
+    }
+}
+
+// Gathers contrib[ci[x0 + i]] for i in [0, cnt) issuing kGatherBatch
+// independent col_idx loads per thread before their gathers, so each thread
+// keeps that many memory requests in flight. col_idx is streamed once, with
+// non-temporal loads (it must not evict the hot contributions from L2).
+constexpr uint32_t kGatherBatch = 8;
+
+template <class Sink>
+__device__ __forceinline__ void gather_batched(const PrSweep& P, uint32_t x0, uint32_t cnt, Sink sink) {
+    const uint32_t T = blockDim.x;
+    for (uint32_t base = 0; base < cnt; base += kGatherBatch * T) {
+        uint32_t u[kGatherBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            const uint32_t i = base + j * T + threadIdx.x;
+            u[j] = i < cnt ? __builtin_nontemporal_load(&P.ci[x0 + i]) : 0u;
+        }
+        float v[kGatherBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            const uint32_t i = base + j * T + threadIdx.x;
+            v[j] = i < cnt ? P.contrib_in[u[j]] : 0.0f;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            const uint32_t i = base + j * T + threadIdx.x;
+            if (i < cnt) sink(i, v[j]);
+        }
+    }
+}
+
+// Items i < n_litems are long-row chunks, the rest row blocks; block b takes
+// items b, b + gridDim, ... so every block gets a mix of equal-sized work.
+__device__ __forceinline__ void pr_stream_body(const PrSweep& P, const PrStreamLayout& Lyt, float add, float d,
+                                               float inv_n, PrPart& part) {
+    __shared__ float s_val[kStreamNnz];
+    __shared__ float s_hi[16], s_lo[16];
+    const uint32_t T = blockDim.x;
+    const uint32_t n_items = Lyt.n_litems + Lyt.n_sitems;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        if (it < Lyt.n_litems) {
+            const uint4 li = Lyt.litems[it];
+            Kahan k;
+            gather_batched(P, li.y, li.z - li.y, [&](uint32_t, float x) { k.add(x); });
+            float hi = k.s, lo = -k.c;
+            block_comp_reduce(hi, lo, s_hi, s_lo);
+            const uint2 lr = Lyt.lrows[li.w];
+            if (threadIdx.x == 0) {
+                if (lr.y == 1) pr_store(P, li.x, add + d * (hi + lo), part, inv_n);
+                else Lyt.lpart[it] = make_float2(hi, lo);  // combined by pr_longfix_body
+            }
+            __syncthreads();
+            continue;
+        }
+        const uint2 si = Lyt.sitems[it - Lyt.n_litems];
+        const uint32_t r0 = si.x, r1 = si.y;
+        const uint32_t x0 = P.rp[r0], x1 = P.rp[r1];
+        gather_batched(P, x0, x1 - x0, [&](uint32_t i, float x) { s_val[i] = x; });
+        __syncthreads();
+        const uint32_t rows = r1 - r0;
+        uint32_t G = T / (rows ? rows : 1);
+        G = G >= 64 ? 64 : (G >= 32 ? 32 : (G >= 16 ? 16 : (G >= 8 ? 8 : (G >= 4 ? 4 : (G >= 2 ? 2 : 1)))));
+        const uint32_t per_round = T / G, rr = threadIdx.x / G, gl = threadIdx.x % G;
+        for (uint32_t rb = 0; rb < rows; rb += per_round) {
+            const uint32_t r = r0 + rb + rr;
+            Kahan k;
+            if (r < r1) {
+                const uint32_t a = P.rp[r] - x0, b = P.rp[r + 1] - x0;
+                for (uint32_t x = a + gl; x < b; x += G) k.add(s_val[x]);
+            }
+            float hi = k.s, lo = -k.c;
+            comp_reduce(hi, lo, G);
+            if (r < r1 && gl == 0) pr_store(P, r, add + d * (hi + lo), part, inv_n);
+        }
+        __syncthreads();
+    }
+}
+
+// Block-reduces the per-thread FP64 partials and stores them at slot `index`
+// (quantity q at partial[q * cap + index]). No fences and no counters: a
+// later kernel in the same stream reduces the slots.
+__device__ __forceinline__ void write_block_partials(const PrPart& part, double* partial, uint32_t cap,
+                                                     uint32_t index) {
+    __shared__ double red[4][16];
+    const uint32_t lane = lane_id(), w = threadIdx.x / kWave, nwv = blockDim.x / kWave;
+    const double a = wave_sum(part.dang), b = wave_sum(part.delta), c = wave_sum(part.sum), m = wave_max(part.maxrel);
+    __syncthreads();
+    if (lane == 0) {
+        red[0][w] = a;
+        red[1][w] = b;
+        red[2][w] = c;
+        red[3][w] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+        for (uint32_t j = 0; j < nwv; ++j) {
+            t0 += red[0][j];
+            t1 += red[1][j];
+            t2 += red[2][j];
+            t3 = red[3][j] > t3 ? red[3][j] : t3;
+        }
+        partial[index] = t0;
+        partial[cap + index] = t1;
+        partial[2 * cap + index] = t2;
+        partial[3 * cap + index] = t3;
+    }
+}
+
 // The sweep body. `add` = (1-d)/n + d*dangling/n, evaluated in FP64 once.
 __device__ __forceinline__ void pr_sweep_body(const PrSweep& P) {
     const uint32_t b = blockIdx.x;
@@ -125,6 +281,7 @@ __device__ __forceinline__ void pr_sweep_body(const PrSweep& P) {
     PrPart part;
     __shared__ float s_hi[16], s_lo[16];
 
+    if (P.use_stream) pr_stream_body(P, P.stream, add, d, inv_n, part);
     for (uint32_t si = 0; si < P.nsec; ++si) {
         const PrSection& S = P.sec[si];
         if (b < S.blk_begin || b >= S.blk_end) continue;
@@ -182,50 +339,51 @@ __device__ __forceinline__ void pr_sweep_body(const PrSweep& P) {
         }
     }
 
-    // ---- block partials, then the last block reduces all partials in order
-    __shared__ double red[4][16];
-    __shared__ bool last;
-    double a = wave_sum(part.dang), bb = wave_sum(part.delta), c = wave_sum(part.sum), mx = wave_max(part.maxrel);
-    const uint32_t w = threadIdx.x / kWave, nwv = blockDim.x / kWave;
-    if (lane == 0) {
-        red[0][w] = a;
-        red[1][w] = bb;
-        red[2][w] = c;
-        red[3][w] = mx;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
-        for (uint32_t j = 0; j < nwv; ++j) {
-            t0 += red[0][j];
-            t1 += red[1][j];
-            t2 += red[2][j];
-            t3 = red[3][j] > t3 ? red[3][j] : t3;
+    write_block_partials(part, P.partial, P.partial_cap, P.partial_base + b);
+}
+
+// Multi-chunk long rows: combine the chunk partials of each row in chunk order
+// (wave per row; lane j takes chunks j, j+64, ...; then a TwoSum butterfly) and
+// store the row. Runs after the sweep, so the partials are visible.
+__device__ __forceinline__ void pr_longfix_body(const PrSweep& P) {
+    const PrStreamLayout& Lyt = P.stream;
+    const double addd = (1.0 - (double)P.damping) / (double)P.n + (double)P.damping * P.acc_in->dangling / (double)P.n;
+    const float add = (float)addd, d = P.damping, inv_n = 1.0f / (float)P.n;
+    const uint32_t lane = lane_id(), wpb = blockDim.x / kWave, wid = threadIdx.x / kWave;
+    PrPart part;
+    for (uint32_t row = blockIdx.x * wpb + wid; row < Lyt.n_lrows; row += gridDim.x * wpb) {
+        const uint2 lr = Lyt.lrows[row];
+        if (lr.y < 2) continue;  // single-chunk rows were stored by the sweep (wave-uniform)
+        float H = 0.f, L = 0.f;
+        for (uint32_t j = lane; j < lr.y; j += kWave) {
+            const float2 q = Lyt.lpart[lr.x + j];
+            float a, e;
+            two_sum(H, q.x, a, e);
+            H = a;
+            L = L + q.y + e;
         }
-        P.partial[b] = t0;
-        P.partial[P.partial_cap + b] = t1;
-        P.partial[2 * P.partial_cap + b] = t2;
-        P.partial[3 * P.partial_cap + b] = t3;
-        __threadfence();
-        const uint32_t t = atomicAdd(P.ticket, 1u);
-        last = t == gridDim.x - 1;
+        comp_reduce(H, L, kWave);
+        if (lane == 0) pr_store(P, Lyt.litems[lr.x].x, add + d * (H + L), part, inv_n);
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    write_block_partials(part, P.partial, P.partial_cap, P.partial_base + blockIdx.x);
+}
+
+// One block: sums partials [0, count) of each quantity in index order.
+__device__ __forceinline__ void pr_reduce_body(const double* partial, uint32_t cap, uint32_t count, PrAcc* out) {
+    __shared__ double red[4][16];
+    const uint32_t lane = lane_id(), w = threadIdx.x / kWave, nwv = blockDim.x / kWave;
     double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
-    for (uint32_t j = threadIdx.x; j < gridDim.x; j += blockDim.x) {
-        t0 += P.partial[j];
-        t1 += P.partial[P.partial_cap + j];
-        t2 += P.partial[2 * P.partial_cap + j];
-        const double m3 = P.partial[3 * P.partial_cap + j];
+    for (uint32_t j = threadIdx.x; j < count; j += blockDim.x) {
+        t0 += partial[j];
+        t1 += partial[cap + j];
+        t2 += partial[2 * cap + j];
+        const double m3 = partial[3 * cap + j];
         t3 = m3 > t3 ? m3 : t3;
     }
     t0 = wave_sum(t0);
     t1 = wave_sum(t1);
     t2 = wave_sum(t2);
     t3 = wave_max(t3);
-    __syncthreads();
     if (lane == 0) {
         red[0][w] = t0;
         red[1][w] = t1;
@@ -241,19 +399,10 @@ __device__ __forceinline__ void pr_sweep_body(const PrSweep& P) {
             T2 += red[2][j];
             T3 = red[3][j] > T3 ? red[3][j] : T3;
         }
-        PrAcc* o = P.acc_out;
-        if (P.first) {
-            o->dangling = T0;
-            o->delta = T1;
-            o->sum = T2;
-            o->maxrel = T3;
-        } else {
-            o->dangling += T0;
-            o->delta += T1;
-            o->sum += T2;
-            o->maxrel = T3 > o->maxrel ? T3 : o->maxrel;
-        }
-        *P.ticket = 0;  // ready for the next launch
+        out->dangling = T0;
+        out->delta = T1;
+        out->sum = T2;
+        out->maxrel = T3;
     }
 }
 

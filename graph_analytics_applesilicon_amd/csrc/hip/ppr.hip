@@ -37,7 +37,6 @@ struct PprArgs {
     double* dang_out;       // B
     double* delta_out;      // 1
     double* partial;        // (B + 1) * kPprBlocks
-    uint32_t* ticket;
     uint32_t n, B;
     float damping;
 };
@@ -106,7 +105,6 @@ __global__ void k_ppr_sweep(PprArgs A) {
     // block partials: per column dangling (waves combined in LDS), total delta
     __shared__ double s_d[16][kWave];
     __shared__ double s_x[16];
-    __shared__ bool last;
     for (uint32_t c = 0; c < chunks; ++c) {
         const uint32_t b = c * kWave + lane;
         __syncthreads();
@@ -126,21 +124,17 @@ __global__ void k_ppr_sweep(PprArgs A) {
         double t = 0;
         for (uint32_t w = 0; w < wpb; ++w) t += s_x[w];
         A.partial[(size_t)blockIdx.x * (A.B + 1) + A.B] = t;
-        __threadfence();
-        last = atomicAdd(A.ticket, 1u) == gridDim.x - 1;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // last block: column sums over blocks in block order (deterministic)
-    for (uint32_t col = threadIdx.x; col <= A.B; col += blockDim.x) {
+}
+
+// One block: column sums of the sweep's block partials, in block order.
+__global__ void k_ppr_reduce(const double* partial, uint32_t blocks, uint32_t B, double* dang_out, double* delta_out) {
+    for (uint32_t col = threadIdx.x; col <= B; col += blockDim.x) {
         double t = 0;
-        for (uint32_t blk = 0; blk < gridDim.x; ++blk) t += A.partial[(size_t)blk * (A.B + 1) + col];
-        if (col < A.B) A.dang_out[col] = t;
-        else *A.delta_out = t;
+        for (uint32_t blk = 0; blk < blocks; ++blk) t += partial[(size_t)blk * (B + 1) + col];
+        if (col < B) dang_out[col] = t;
+        else *delta_out = t;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) *A.ticket = 0;
 }
 
 }  // namespace ggpu
@@ -164,10 +158,6 @@ ggpu_pr_result ppr_run(PrEngine& e, const uint32_t* seeds, uint32_t B, const ggp
     S.dang.alloc(2 * kPprMaxB);
     S.delta.alloc(2);
     S.seeds.alloc(kPprMaxB);
-    if (!S.ticket.p) {
-        S.ticket.alloc(1);
-        GGPU_HIP_TRY(hipMemsetAsync(S.ticket.p, 0, 4, s));
-    }
     while (S.ev.size() < 2) {
         hipEvent_t ev;
         GGPU_HIP_TRY(hipEventCreate(&ev));
@@ -205,11 +195,12 @@ ggpu_pr_result ppr_run(PrEngine& e, const uint32_t* seeds, uint32_t B, const ggp
         A.dang_out = S.dang.p + (size_t)nx * kPprMaxB;
         A.delta_out = S.delta.p;
         A.partial = S.part.p;
-        A.ticket = S.ticket.p;
         A.n = n;
         A.B = B;
         A.damping = o.damping;
         k_ppr_sweep<<<grid, o.tg, 0, s>>>(A);
+        GGPU_HIP_TRY(hipGetLastError());
+        k_ppr_reduce<<<1, 256, 0, s>>>(S.part.p, grid, B, A.dang_out, A.delta_out);
         GGPU_HIP_TRY(hipGetLastError());
         if (!tol_mode) continue;
         GGPU_HIP_TRY(hipMemcpyAsync(pin, S.delta.p, 8, hipMemcpyDeviceToHost, s));

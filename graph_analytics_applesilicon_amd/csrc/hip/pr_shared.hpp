@@ -20,7 +20,7 @@ struct PprView {
 struct PprState {
     DBuf<float> r, c;
     DBuf<double> part, dang, delta;
-    DBuf<uint32_t> seeds, ticket;
+    DBuf<uint32_t> seeds;
     std::vector<hipEvent_t> ev;
     uint32_t B = 0;
     int slot = 0;

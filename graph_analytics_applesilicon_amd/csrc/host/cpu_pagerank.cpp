@@ -84,7 +84,7 @@ bool gs_no_worse_than_jacobi(const PrCheck& gs, const PrCheck& jacobi) {
            gs.max_rel_top <= std::max(1e-3, jacobi.max_rel_top * (1 + 1e-4));
 }
 
-GsOrder gs_order(const Csr& in, const uint32_t* outdeg, uint32_t pair_max_outdeg) {
+GsOrder gs_order(const Csr& in, const uint32_t* outdeg, uint32_t pair_max_outdeg, bool allow_colored) {
     constexpr int kRounds = 8;
     constexpr double kCoverage = 0.95;
     const uint32_t n = (uint32_t)in.n;
@@ -98,7 +98,7 @@ GsOrder gs_order(const Csr& in, const uint32_t* outdeg, uint32_t pair_max_outdeg
     std::vector<uint8_t> color(n, 0xFF);
     std::vector<std::vector<uint32_t>> sets;
     uint64_t covered = 0;
-    for (int c = 0; c < kRounds && covered < n; ++c) {
+    for (int c = 0; allow_colored && c < kRounds && covered < n; ++c) {
         std::vector<uint8_t> pick(n, 0);
         // A vertex joins this round's set if it beats every uncoloured in-neighbour
         // (decided on the colouring state at the start of the round).

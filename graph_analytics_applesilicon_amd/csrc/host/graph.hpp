@@ -136,6 +136,7 @@ struct GsOrder {
     // natural order with pair deferral: main list then deferred list (old ids)
     std::vector<uint32_t> main_list, deferred_list;
 };
-GsOrder gs_order(const Csr& in, const uint32_t* outdeg, uint32_t pair_max_outdeg);
+// allow_colored = false skips the colouring and returns the natural order.
+GsOrder gs_order(const Csr& in, const uint32_t* outdeg, uint32_t pair_max_outdeg, bool allow_colored = true);
 
 }  // namespace ggpu

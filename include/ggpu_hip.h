@@ -127,8 +127,20 @@ typedef struct ggpu_pr_opts {
     uint32_t small_max;    /* adaptive: in-degree <  small_max -> lane per vertex */
     uint32_t large_min;    /* adaptive: in-degree >= large_min -> workgroup per vertex */
     int in_place;          /* Jacobi variants: update contributions in place (Gauss-Seidel style) */
-    uint32_t medium_lanes; /* adaptive: lanes per medium vertex (4, 8, 16, 32 or 64) */
+    uint32_t medium_lanes; /* adaptive bins: lanes per medium vertex (power of two <= 64) */
+    uint32_t layout;       /* gpu_pull_adaptive work layout: GGPU_PR_LAYOUT_* */
+    uint32_t order;        /* vertex numbering the sweeps run in: GGPU_PR_ORDER_* (ranks come back in caller ids) */
 } ggpu_pr_opts;
+/* GIVEN: the caller's ids. DEGREE: an internal copy relabeled by descending
+ * out-degree, so the most-gathered contributions share cache lines. */
+#define GGPU_PR_ORDER_GIVEN 0
+#define GGPU_PR_ORDER_DEGREE 1
+/* STREAM: consecutive short in-lists packed into ~2K-edge row blocks whose
+ * gathers are staged in LDS, long lists split into chunks (default).
+ * BINS: the reference's shape, one lane / lane group / workgroup per vertex by
+ * in-degree (small_max, large_min, medium_lanes). */
+#define GGPU_PR_LAYOUT_STREAM 0
+#define GGPU_PR_LAYOUT_BINS 1
 void ggpu_pr_opts_default(ggpu_pr_opts* o);
 /* "gpu_pull_scalar" | "gpu_pull_vector" | "gpu_pull_adaptive" | "gpu_pull_gs" -> GGPU_PR_* */
 int ggpu_pr_parse_variant(const char* name, int* variant);

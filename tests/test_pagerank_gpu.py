@@ -24,13 +24,19 @@ def _in_graph(name):
     return g, t, g.degrees()
 
 
-CFGS = [("gpu_pull_scalar", {}), ("gpu_pull_vector", {}), ("gpu_pull_adaptive", {}),
-        ("gpu_pull_adaptive", dict(small_max=1 << 30, large_min=1 << 31)),  # all lane-per-vertex
-        ("gpu_pull_adaptive", dict(small_max=0, large_min=0)),              # all workgroup-per-vertex
-        ("gpu_pull_adaptive", dict(small_max=0, large_min=1 << 30)),        # all medium
-        ("gpu_pull_adaptive", dict(small_max=2, large_min=3)),
-        ("gpu_pull_adaptive", dict(small_max=4, large_min=64, medium_lanes=8)),
-        ("gpu_pull_adaptive", dict(small_max=8, large_min=512, medium_lanes=16))]
+B = G.PR_LAYOUT_BINS
+CFGS = [("gpu_pull_scalar", {}), ("gpu_pull_vector", {}),
+        ("gpu_pull_adaptive", {}),                                          # stream layout (default)
+        ("gpu_pull_adaptive", dict(order=G.PR_ORDER_DEGREE)),               # stream on the degree-ordered copy
+        ("gpu_pull_adaptive", dict(order=G.PR_ORDER_DEGREE, layout=1)),
+        ("gpu_pull_scalar", dict(order=G.PR_ORDER_DEGREE)),
+        ("gpu_pull_adaptive", dict(layout=B)),                              # the reference's 16 / 256 bins
+        ("gpu_pull_adaptive", dict(layout=B, small_max=1 << 30, large_min=1 << 31)),  # all lane-per-vertex
+        ("gpu_pull_adaptive", dict(layout=B, small_max=0, large_min=0)),              # all workgroup-per-vertex
+        ("gpu_pull_adaptive", dict(layout=B, small_max=0, large_min=1 << 30)),        # all medium
+        ("gpu_pull_adaptive", dict(layout=B, small_max=2, large_min=3)),
+        ("gpu_pull_adaptive", dict(layout=B, small_max=4, large_min=64, medium_lanes=8)),
+        ("gpu_pull_adaptive", dict(layout=B, small_max=8, large_min=512, medium_lanes=16))]
 
 
 @pytest.mark.parametrize("name", SUITE)
@@ -89,10 +95,14 @@ def test_pagerank_gs_converges_to_oracle():
             natural += not is_col
             deferred_graphs += dfr > 0
             for tg in (64, 256, 1024):
-                for what, tol, tol_rel, iters in (("20 sweeps", 0.0, 0.0, 20), ("200 sweeps", 0.0, 0.0, 200),
-                                                  ("tol 2.5e-7", 2.5e-7, 0.0, 1000),
-                                                  ("tol 1e-6 + rel", 1e-6, 2.5e-5, 1000)):
-                    res = pr.run("gpu_pull_gs", G.pr_options(iters=iters, tol=tol, tol_rel=tol_rel, tg=tg))
+                for what, tol, tol_rel, iters, kw in (("20 sweeps", 0.0, 0.0, 20, {}),
+                                                      ("200 sweeps", 0.0, 0.0, 200, {}),
+                                                      ("200 sweeps bins", 0.0, 0.0, 200, dict(layout=1)),
+                                                      ("200 sweeps degree", 0.0, 0.0, 200, dict(order=1)),
+                                                      ("tol 2.5e-7", 2.5e-7, 0.0, 1000, {}),
+                                                      ("tol 2.5e-7 bins", 2.5e-7, 0.0, 1000, dict(layout=1)),
+                                                      ("tol 1e-6 + rel", 1e-6, 2.5e-5, 1000, {})):
+                    res = pr.run("gpu_pull_gs", G.pr_options(iters=iters, tol=tol, tol_rel=tol_rel, tg=tg, **kw))
                     c = O.check_ranks(conv, pr.ranks())
                     if iters == 20:
                         ok = abs(c[2] - 1.0) < 1e-4
@@ -118,10 +128,11 @@ def test_inplace_jacobi_variants_converge(name):
     g, t, od = _in_graph(name)
     conv, _, _ = O.pagerank(t.row_ptr, t.col_idx, od, 0.85, 1000, 1e-10)
     pr = G.GpuPageRank(t, od)
-    for v in ("gpu_pull_scalar", "gpu_pull_vector", "gpu_pull_adaptive"):
-        pr.run(v, G.pr_options(iters=200, in_place=1))
+    for v, kw in (("gpu_pull_scalar", {}), ("gpu_pull_vector", {}), ("gpu_pull_adaptive", {}),
+                  ("gpu_pull_adaptive", dict(layout=G.PR_LAYOUT_BINS))):
+        pr.run(v, G.pr_options(iters=200, in_place=1, **kw))
         c = O.check_ranks(conv, pr.ranks())
-        assert O.ranks_ok(*c), (v, c)
+        assert O.ranks_ok(*c), (v, kw, c)
 
 
 def test_pr_errors_are_reported():
@@ -161,3 +172,28 @@ def test_ppr_isolated_seed_is_one_hot():
     assert np.allclose(blk[:, 0], [0, 0, 0, 1, 0]) and np.allclose(blk[:, 1], [0, 0, 0, 0, 1])
     assert abs(blk[:, 2].sum() - 1) < 1e-6
     del g
+
+
+def test_stream_layout_long_rows_and_autotune():
+    """In-lists longer than a row block are split into chunks and recombined in
+    chunk order: the star's hub (in-degree 100000 = 13 chunks) and a 2-chunk
+    row must match the oracle; the autotuner's pick must leave ranks valid."""
+    for name in ("star100k", "rmat12_directed"):
+        g, t, od = _in_graph(name)
+        ref, _, _ = O.pagerank(t.row_ptr, t.col_idx, od, 0.85, 20, 0.0)
+        pr = G.GpuPageRank(t, od)
+        for tg in (64, 256, 1024):
+            pr.run("gpu_pull_adaptive", G.pr_options(iters=20, tg=tg))
+            assert O.ranks_ok(*O.check_ranks(ref, pr.ranks())), (name, tg)
+        o = pr.autotune(iters=3)
+        pr.run("gpu_pull_adaptive", G.pr_options(iters=20, layout=o.layout, small_max=o.small_max,
+                                                 large_min=o.large_min, medium_lanes=o.medium_lanes))
+        assert O.ranks_ok(*O.check_ranks(ref, pr.ranks())), (name, o.layout, o.small_max, o.large_min)
+    # a hub with 10000 in-neighbours: 2 chunks of the long-row path
+    edges = [(i, 0) for i in range(1, 10001)] + [(0, 1)]
+    g = G.Graph.from_edges(10001, edges, directed=True)
+    t = g.transpose()
+    ref, _, _ = O.pagerank(t.row_ptr, t.col_idx, g.degrees(), 0.85, 50, 0.0)
+    pr = G.GpuPageRank(t, g.degrees())
+    pr.run("gpu_pull_adaptive", G.pr_options(iters=50))
+    assert O.ranks_ok(*O.check_ranks(ref, pr.ranks()))
