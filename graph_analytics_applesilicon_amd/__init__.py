@@ -69,7 +69,8 @@ class BfsOpts(C.Structure):
 
 class MsbfsResult(C.Structure):
     _fields_ = [("levels", C.c_uint32), ("pad", C.c_uint32), ("gpu_ms", C.c_double),
-                ("level_actions", C.c_uint64 * 3)]
+                ("level_actions", C.c_uint64 * 3), ("setup_ms", C.c_double), ("levels_ms", C.c_double),
+                ("finish_ms", C.c_double)]
 
 
 class PrOpts(C.Structure):

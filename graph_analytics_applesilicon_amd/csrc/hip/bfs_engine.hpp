@@ -177,10 +177,9 @@ struct BfsEngine {
     DBuf<uint8_t> dactions;
     DBuf<ggpu_level> dtrace;
     DBuf<unsigned int> barrier;
-    // batched BFS resources (msbfs.hip)
-    DBuf<uint64_t> ms_masks;
-    DBuf<uint32_t> ms_dist, ms_list;
-    uint32_t ms_k = 0, ms_levels = 0;
+    // batched BFS state (msbfs.hip), released through ms_free
+    void* ms_state = nullptr;
+    void (*ms_free)(void*) = nullptr;
 
     BfsEngine(ggpu_ctx* c, uint64_t n, uint64_t m, const uint32_t* rp, const uint32_t* ci);
     ~BfsEngine();

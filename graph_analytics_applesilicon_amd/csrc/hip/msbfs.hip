@@ -1,0 +1,880 @@
+// Batched multi-source BFS (no reference counterpart): k <= 256 sources
+// traverse together in ONE persistent cooperative launch, one grid barrier
+// per level.
+//
+// State, W = ceil(k/64) u64 words per vertex, bit s%64 of word s/64 = source s:
+//   visit[v][w]       sources that have reached v
+//   F_l[v][w]         sources for which v is at depth l       (C level slots)
+//   fb_l[v/64]        bit v%64: F_l[v] != 0 (the level-l frontier)
+// Level l reads F_l / fb_l and writes F_{l+1} / fb_{l+1} / visit:
+//   bottom-up: an unfinished vertex ORs the F_l words of its frontier
+//     neighbours until every bit it still needs is present;
+//   top-down: a frontier vertex ORs F_l[v] & ~visit[u] into next[u] of its
+//     neighbours; after a barrier each touched vertex keeps next & ~visit
+//     as its new bits, so each (source, vertex) pair is discovered once.
+// Vertices of degree >= kMsHeavy are scanned / pushed as kMsChunk-edge items
+// by whole waves (both directions), the rest lane per vertex.
+// Depths are not written during the traversal: the level slots are turned
+// into depths through bit planes (plane p = OR of F over levels whose offset
+// has bit p) and written source-major with coalesced stores, when the C
+// slots are used up and at the end. That pass also clears what it read, so
+// visit / next / F / fb are all zero between launches.
+//
+// Cross-block data written inside a level goes through device-scope atomics
+// or write-through stores only (grid_sync.hpp), and every decision input
+// (per-block counters) is reduced identically by every block, so all blocks
+// take the same branch at every level without a second barrier.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "bfs_engine.hpp"
+#include "grid_sync.hpp"
+
+namespace ggpu {
+
+constexpr uint32_t kMsThreads = 1024;
+constexpr uint32_t kMsHeavy = 32;       // degree from which a vertex is handled as wave items
+constexpr uint32_t kMsChunk = 1024;     // edges per item
+constexpr uint32_t kMsSlots = 16;       // level slots (<= 64: six depth planes)
+constexpr uint32_t kMsTraceCap = 1u << 16;
+constexpr uint32_t kPartWords = 4;
+
+struct MsDev {
+    const uint32_t* rp;
+    const uint32_t* ci;
+    uint64_t* visit;               // n * W
+    uint64_t* next;                // n * W, top-down push target (zero between levels)
+    uint64_t* F;                   // C * n * W
+    uint64_t* fb;                  // C * nb
+    unsigned long long* part;      // 2 * G * kPartWords
+    uint32_t* dist;                // k * n, source-major
+    const uint32_t* sources;       // k
+    const uint4* heavy;            // {v, first edge, end edge, degree}
+    uint32_t* bar;                 // kBarWords
+    uint32_t* abort;
+    uint32_t* out;                 // {levels, 0, actions[3], 0, clock at start, level 0, loop end, finish}
+    ggpu_level* trace;             // trace_cap entries
+    const uint8_t* actions;        // scripted policy
+    const float* mlp;              // learned policy parameters or null
+    double alpha, beta;
+    double tick_ms;                // wall_clock64() period in ms (per-level times in the trace)
+    uint64_t m;
+    uint64_t valid_last;           // valid bits of the last source word
+    uint32_t n, k, nb, n_heavy, C, trace_cap, n_actions, policy;
+};
+
+struct MsAcc {
+    unsigned long long size = 0, edges = 0, pairs = 0, examined = 0;
+};
+
+template <int W>
+__device__ __forceinline__ uint64_t valid_word(const MsDev& D, int w) {
+    return w == W - 1 ? D.valid_last : ~0ull;
+}
+
+__device__ __forceinline__ uint32_t gwave() { return (blockIdx.x * blockDim.x + threadIdx.x) / kWave; }
+__device__ __forceinline__ uint32_t nwaves() { return gridDim.x * blockDim.x / kWave; }
+
+// Per-block counters of the level being produced, one write-through store each.
+__device__ void ms_flush(const MsDev& D, uint32_t slot, const MsAcc& a) {
+    __shared__ unsigned long long s_red[kMsThreads / kWave];
+    const unsigned long long v[kPartWords] = {a.size, a.edges, a.pairs, a.examined};
+    unsigned long long t[kPartWords];
+#pragma unroll
+    for (uint32_t j = 0; j < kPartWords; ++j) t[j] = block_sum(v[j], s_red);
+    if (threadIdx.x == 0) {
+        unsigned long long* p = D.part + ((size_t)slot * gridDim.x + blockIdx.x) * kPartWords;
+#pragma unroll
+        for (uint32_t j = 0; j < kPartWords; ++j) store_through(&p[j], t[j]);
+    }
+}
+
+// Totals of one counter slot, in every thread.
+__device__ MsAcc ms_totals(const MsDev& D, uint32_t slot) {
+    __shared__ unsigned long long s_red[kMsThreads / kWave];
+    unsigned long long v[kPartWords] = {0, 0, 0, 0};
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+        const unsigned long long* p = D.part + ((size_t)slot * gridDim.x + i) * kPartWords;
+#pragma unroll
+        for (uint32_t j = 0; j < kPartWords; ++j) v[j] += p[j];
+    }
+    MsAcc a;
+    a.size = block_sum(v[0], s_red);
+    a.edges = block_sum(v[1], s_red);
+    a.pairs = block_sum(v[2], s_red);
+    a.examined = block_sum(v[3], s_red);
+    return a;
+}
+
+// Marks bits x of vertex u as reached at the level being produced; only the
+// bits that atomicOr on visit reports as new count, so concurrent
+// discoverers of the same pair count it once.
+template <int W>
+__device__ __forceinline__ void ms_discover(const MsDev& D, uint64_t* Fn, uint64_t* fbn, uint32_t u,
+                                            const uint64_t (&x)[W], MsAcc& a) {
+    uint32_t pc = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        if (!x[w]) continue;
+        const uint64_t old = atomicOr((unsigned long long*)&D.visit[(size_t)u * W + w], (unsigned long long)x[w]);
+        const uint64_t fresh = x[w] & ~old;
+        if (fresh) {
+            atomicOr((unsigned long long*)&Fn[(size_t)u * W + w], (unsigned long long)fresh);
+            pc += (uint32_t)__popcll(fresh);
+        }
+    }
+    if (pc) {
+        const uint64_t dg = D.rp[u + 1] - D.rp[u];
+        a.pairs += dg * pc;
+        const uint64_t bit = 1ull << (u & 63);
+        const uint64_t old = atomicOr((unsigned long long*)&fbn[u >> 6], (unsigned long long)bit);
+        if (!(old & bit)) {
+            a.size += 1;
+            a.edges += dg;
+        }
+    }
+}
+
+template <int W>
+__device__ void ms_seed(const MsDev& D, MsAcc& a) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < D.k) {
+        uint64_t x[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) x[w] = (uint32_t)w == t / 64 ? 1ull << (t % 64) : 0ull;
+        ms_discover<W>(D, D.F, D.fb, D.sources[t], x, a);
+    }
+}
+
+// ---------------------------------------------------------------- bottom-up
+template <int W>
+__device__ void ms_bottom_up(const MsDev& D, uint32_t level, MsAcc& a) {
+    const uint32_t cs = level % D.C, ns = (level + 1) % D.C;
+    const uint64_t* F = D.F + (size_t)cs * D.n * W;
+    const uint64_t* fb = D.fb + (size_t)cs * D.nb;
+    uint64_t* Fn = D.F + (size_t)ns * D.n * W;
+    uint64_t* fbn = D.fb + (size_t)ns * D.nb;
+    const uint32_t lane = lane_id();
+
+    // light vertices: lane per vertex, 64 consecutive vertices per wave
+    for (uint32_t wd = gwave(); wd < D.nb; wd += nwaves()) {
+        const uint32_t v = wd * 64 + lane;
+        bool found = false;
+        if (v < D.n) {
+            const uint32_t b = D.rp[v], e = D.rp[v + 1];
+            const uint32_t dg = e - b;
+            if (dg > 0 && dg < kMsHeavy) {
+                uint64_t need[W], acc[W];
+                bool any = false;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    need[w] = ~D.visit[(size_t)v * W + w] & valid_word<W>(D, w);
+                    acc[w] = 0;
+                    any |= need[w] != 0;
+                }
+                if (any) {
+                    for (uint32_t x = b; x < e; x += 4) {
+                        const uint32_t c = min(4u, e - x);
+                        uint32_t u[4];
+                        uint64_t f[4];
+#pragma unroll
+                        for (uint32_t i = 0; i < 4; ++i) u[i] = i < c ? D.ci[x + i] : 0u;
+#pragma unroll
+                        for (uint32_t i = 0; i < 4; ++i) f[i] = i < c ? fb[u[i] >> 6] >> (u[i] & 63) : 0ull;
+#pragma unroll
+                        for (uint32_t i = 0; i < 4; ++i)
+                            if (f[i] & 1ull) {
+#pragma unroll
+                                for (int w = 0; w < W; ++w) acc[w] |= F[(size_t)u[i] * W + w];
+                            }
+                        a.examined += c;
+                        bool all = true;
+#pragma unroll
+                        for (int w = 0; w < W; ++w) all &= (acc[w] & need[w]) == need[w];
+                        if (all) break;
+                    }
+                    uint32_t pc = 0;
+                    uint64_t nw[W];
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        nw[w] = acc[w] & need[w];
+                        pc += (uint32_t)__popcll(nw[w]);
+                    }
+                    if (pc) {
+                        // a light vertex is scanned by this lane only: its bits are new
+                        found = true;
+#pragma unroll
+                        for (int w = 0; w < W; ++w)
+                            if (nw[w]) {
+                                atomicOr((unsigned long long*)&D.visit[(size_t)v * W + w], (unsigned long long)nw[w]);
+                                atomicOr((unsigned long long*)&Fn[(size_t)v * W + w], (unsigned long long)nw[w]);
+                            }
+                        a.pairs += (uint64_t)dg * pc;
+                        a.edges += dg;
+                    }
+                }
+            }
+        }
+        const uint64_t mask = __ballot(found);
+        if (lane == 0 && mask) {
+            atomicOr((unsigned long long*)&fbn[wd], (unsigned long long)mask);
+            a.size += (uint64_t)__popcll(mask);
+        }
+    }
+
+    // heavy items: the lanes of a wave look at 64 items nwaves apart (the
+    // chunks of one vertex land on different waves), the wave then scans each
+    // item that still needs bits, 256 edges at a time
+    for (uint64_t r = 0; gwave() + r * 64 * nwaves() < D.n_heavy; ++r) {
+        const uint64_t it = gwave() + (r * 64 + lane) * nwaves();
+        uint4 item = make_uint4(0, 0, 0, 0);
+        uint64_t need[W];
+        bool want = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) need[w] = 0;
+        if (it < D.n_heavy) {
+            item = D.heavy[it];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                need[w] = ~D.visit[(size_t)item.x * W + w] & valid_word<W>(D, w);
+                want |= need[w] != 0;
+            }
+        }
+        uint64_t todo = __ballot(want);
+        while (todo) {
+            const int j = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            const uint32_t v = __shfl(item.x, j, kWave), y = __shfl(item.y, j, kWave), z = __shfl(item.z, j, kWave);
+            uint64_t nd[W], acc[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                nd[w] = __shfl(need[w], j, kWave);
+                acc[w] = 0;
+            }
+            for (uint32_t xb = y; xb < z; xb += 4 * kWave) {
+                uint32_t u[4];
+                uint64_t f[4];
+#pragma unroll
+                for (uint32_t i = 0; i < 4; ++i) {
+                    const uint32_t x = xb + i * kWave + lane;
+                    u[i] = x < z ? D.ci[x] : 0xFFFFFFFFu;
+                }
+#pragma unroll
+                for (uint32_t i = 0; i < 4; ++i) f[i] = u[i] != 0xFFFFFFFFu ? fb[u[i] >> 6] >> (u[i] & 63) : 0ull;
+#pragma unroll
+                for (uint32_t i = 0; i < 4; ++i) {
+                    if (f[i] & 1ull) {
+#pragma unroll
+                        for (int w = 0; w < W; ++w) acc[w] |= F[(size_t)u[i] * W + w];
+                    }
+                    a.examined += u[i] != 0xFFFFFFFFu;
+                }
+                bool all = true;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    uint64_t r = acc[w];
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) r |= __shfl_xor(r, off, kWave);
+                    acc[w] = r;
+                    all &= (r & nd[w]) == nd[w];
+                }
+                if (all) break;
+            }
+            if (lane == 0) {
+                uint64_t x[W];
+                bool any = false;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    x[w] = acc[w] & nd[w];
+                    any |= x[w] != 0;
+                }
+                if (any) ms_discover<W>(D, Fn, fbn, v, x, a);
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------- top-down
+// Two phases. Push: every frontier edge v -> u ORs F_l[v] & ~visit[u] into
+// next[u] and sets u's bit in fb_{l+1} (fire-and-forget atomics; visit is
+// not written, so the filter is exact for earlier levels and duplicates of
+// this level merge in next). Collect (after a barrier): each marked vertex
+// takes its new bits, clears next, and fb_{l+1} is rewritten exactly.
+template <int W>
+__device__ __forceinline__ void ms_push4(const MsDev& D, uint64_t* fbn, const uint32_t (&u)[4], uint32_t c,
+                                         const uint64_t (&f)[W]) {
+    uint64_t vis[4][W];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i)
+#pragma unroll
+        for (int w = 0; w < W; ++w) vis[i][w] = (i < c && f[w]) ? D.visit[(size_t)u[i] * W + w] : ~0ull;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        bool any = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint64_t x = f[w] & ~vis[i][w];
+            if (x) {
+                atomicOr((unsigned long long*)&D.next[(size_t)u[i] * W + w], (unsigned long long)x);
+                any = true;
+            }
+        }
+        if (any) atomicOr((unsigned long long*)&fbn[u[i] >> 6], (unsigned long long)(1ull << (u[i] & 63)));
+    }
+}
+
+// Pushes edges [b, e) of a vertex with frontier bits f, `step` lanes apart.
+template <int W>
+__device__ __forceinline__ void ms_push_range(const MsDev& D, uint64_t* fbn, uint32_t b, uint32_t e, uint32_t first,
+                                              uint32_t step, const uint64_t (&f)[W]) {
+    for (uint32_t x = b + first; x < e; x += 4 * step) {
+        uint32_t u[4];
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            const uint32_t y = x + i * step;
+            u[i] = y < e ? D.ci[y] : 0u;
+            c += y < e;
+        }
+        ms_push4<W>(D, fbn, u, c, f);
+    }
+}
+
+template <int W>
+__device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex) {
+    const uint32_t cs = level % D.C, ns = (level + 1) % D.C;
+    const uint64_t* F = D.F + (size_t)cs * D.n * W;
+    const uint64_t* fb = D.fb + (size_t)cs * D.nb;
+    uint64_t* fbn = D.fb + (size_t)ns * D.nb;
+    const uint32_t lane = lane_id();
+    const uint32_t stride = gridDim.x * blockDim.x;
+
+    // light frontier vertices: each lane owns a frontier word and, round by
+    // round, takes its next set bit, so 64 vertices of 64 words run side by side
+    for (uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x; wi - lane < D.nb; wi += stride) {
+        uint64_t fw = wi < D.nb ? fb[wi] : 0ull;
+        while (__ballot(fw != 0)) {
+            uint32_t v = 0xFFFFFFFFu;
+            if (fw) {
+                v = wi * 64 + (uint32_t)(__ffsll((long long)fw) - 1);
+                fw &= fw - 1;
+            }
+            if (lane_per_vertex) {
+                // binned: the lane pushes its vertex
+                if (v != 0xFFFFFFFFu) {
+                    const uint32_t b = D.rp[v], e = D.rp[v + 1];
+                    if (e - b < kMsHeavy) {
+                        uint64_t f[W];
+#pragma unroll
+                        for (int w = 0; w < W; ++w) f[w] = F[(size_t)v * W + w];
+                        ms_push_range<W>(D, fbn, b, e, 0, 1, f);
+                    }
+                }
+            } else {
+                // edge: the wave pushes the round's vertices one after another
+                uint32_t b = 0, e = 0;
+                if (v != 0xFFFFFFFFu) {
+                    b = D.rp[v];
+                    e = D.rp[v + 1];
+                }
+                uint64_t todo = __ballot(v != 0xFFFFFFFFu && e - b < kMsHeavy);
+                while (todo) {
+                    const int j = __ffsll((long long)todo) - 1;
+                    todo &= todo - 1;
+                    const uint32_t vj = __shfl(v, j, kWave), bj = __shfl(b, j, kWave), ej = __shfl(e, j, kWave);
+                    uint64_t f[W];
+#pragma unroll
+                    for (int w = 0; w < W; ++w) f[w] = F[(size_t)vj * W + w];
+                    ms_push_range<W>(D, fbn, bj, ej, lane, kWave, f);
+                }
+            }
+        }
+    }
+    // heavy frontier vertices: lanes select, among 64 items nwaves apart, those
+    // whose vertex is in the frontier; the wave pushes each in turn
+    for (uint64_t r = 0; gwave() + r * 64 * nwaves() < D.n_heavy; ++r) {
+        const uint64_t it = gwave() + (r * 64 + lane) * nwaves();
+        uint4 item = make_uint4(0, 0, 0, 0);
+        bool want = false;
+        if (it < D.n_heavy) {
+            item = D.heavy[it];
+            want = (fb[item.x >> 6] >> (item.x & 63)) & 1ull;
+        }
+        uint64_t todo = __ballot(want);
+        while (todo) {
+            const int j = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            const uint32_t v = __shfl(item.x, j, kWave), y = __shfl(item.y, j, kWave), z = __shfl(item.z, j, kWave);
+            uint64_t f[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) f[w] = F[(size_t)v * W + w];
+            ms_push_range<W>(D, fbn, y, z, lane, kWave, f);
+        }
+    }
+}
+
+template <int W>
+__device__ void ms_td_collect(const MsDev& D, uint32_t level, MsAcc& a) {
+    const uint32_t ns = (level + 1) % D.C;
+    uint64_t* Fn = D.F + (size_t)ns * D.n * W;
+    uint64_t* fbn = D.fb + (size_t)ns * D.nb;
+    const uint32_t lane = lane_id();
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // each lane owns one marked word and settles its vertices round by round
+    for (uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x; wi - lane < D.nb; wi += stride) {
+        const uint64_t marked = wi < D.nb ? fbn[wi] : 0ull;
+        uint64_t mk = marked, found = 0;
+        while (__ballot(mk != 0)) {
+            if (mk) {
+                const uint32_t j = (uint32_t)(__ffsll((long long)mk) - 1);
+                mk &= mk - 1;
+                const uint32_t u = wi * 64 + j;
+                uint32_t pc = 0;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    const size_t i = (size_t)u * W + w;
+                    const uint64_t vis = D.visit[i];
+                    const uint64_t nw = D.next[i] & ~vis;
+                    store_through(&D.next[i], 0ull);
+                    if (nw) {
+                        store_through(&Fn[i], nw);
+                        store_through(&D.visit[i], vis | nw);
+                        pc += (uint32_t)__popcll(nw);
+                    }
+                }
+                if (pc) {
+                    const uint64_t dg = D.rp[u + 1] - D.rp[u];
+                    a.pairs += dg * pc;
+                    a.edges += dg;
+                    found |= 1ull << j;
+                }
+            }
+        }
+        if (marked) {
+            store_through(&fbn[wi], found);
+            a.size += (uint64_t)__popcll(found);
+        }
+    }
+}
+
+// ---------------------------------------------------------- depths, cleanup
+// Levels [base, base + cnt) are in slots (base + i) % C. Writes the depth of
+// every pair first reached in them; final: also INF for pairs never reached,
+// and visit cleared. The slots read are cleared for reuse with full-line
+// stores (masked partial-line clears cost several times more): write-through
+// inside the traversal, plain in the final pass (the launch's end writes the
+// caches back).
+template <int W>
+__device__ __forceinline__ void ms_clear(uint64_t* p, bool final) {
+    if (final) *p = 0ull;
+    else store_through(p, 0ull);
+}
+
+template <int W>
+__device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool final) {
+    const uint32_t lane = lane_id();
+    const uint32_t stride = gridDim.x * blockDim.x;  // a multiple of 64: lane 0 owns v % 64 == 0
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < D.n; v += stride) {
+        const uint32_t wd = v >> 6;
+        uint64_t levels = 0;  // bit i: v is in the frontier of level base + i
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const uint32_t s = (base + i) % D.C;
+            if ((D.fb[(size_t)s * D.nb + wd] >> (v & 63)) & 1ull) levels |= 1ull << i;
+        }
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            uint64_t plane[6] = {0, 0, 0, 0, 0, 0}, reached = 0;
+            // uniform over the window's levels: each load is one coalesced
+            // (masked) access across the wave's 64 consecutive vertices
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const uint64_t f = ((levels >> i) & 1ull) ? D.F[((size_t)((base + i) % D.C) * D.n + v) * W + w] : 0ull;
+                reached |= f;
+#pragma unroll
+                for (int p = 0; p < 6; ++p)
+                    if ((i >> p) & 1u) plane[p] |= f;
+            }
+            uint64_t unreached = 0;
+            if (final) {
+                unreached = ~D.visit[(size_t)v * W + w];
+                D.visit[(size_t)v * W + w] = 0ull;
+            }
+            if (!(reached | unreached)) continue;
+            const uint32_t s0 = (uint32_t)w * 64;
+            const uint32_t ns = min(64u, D.k - s0);
+            for (uint32_t b = 0; b < ns; ++b) {
+                uint32_t d;
+                if ((reached >> b) & 1ull) {
+                    d = base;
+#pragma unroll
+                    for (int p = 0; p < 6; ++p) d += (uint32_t)((plane[p] >> b) & 1ull) << p;
+                } else if ((unreached >> b) & 1ull) {
+                    d = GGPU_INF;
+                } else {
+                    continue;
+                }
+                D.dist[(size_t)(s0 + b) * D.n + v] = d;
+            }
+        }
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const size_t slot = (base + i) % D.C;
+#pragma unroll
+            for (int w = 0; w < W; ++w) ms_clear<W>(&D.F[(slot * D.n + v) * W + w], final);
+            // the whole wave has read the frontier word of its 64-vertex group
+            if (lane == 0) ms_clear<W>(&D.fb[slot * D.nb + wd], final);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- decisions
+struct MsPolicyState {
+    uint32_t bottom_up = 0;
+    uint64_t prev_size = 0;
+    unsigned long long cum_pairs = 0;
+};
+
+// Out of line: the MLP's hidden layers stay out of the traversal's registers.
+__device__ __noinline__ uint32_t ms_choose(const MsDev& D, uint32_t level, const MsAcc& t, uint64_t pairs_u,
+                                           MsPolicyState& ps) {
+    uint32_t act;
+    const double kk = (double)D.k;
+    if (D.policy == GGPU_POLICY_SCRIPTED) {
+        act = D.actions[level % D.n_actions];
+        if (act > GGPU_ACTION_BU) act = GGPU_ACTION_BU;
+        ps.bottom_up = act == GGPU_ACTION_BU;
+    } else if (D.policy == GGPU_POLICY_LEARNED && D.mlp) {
+        float f[5], y[3];
+        policy_features(t.size, (uint64_t)((double)t.pairs / kk), (uint64_t)((double)pairs_u / kk), level,
+                        D.n ? (double)D.m / (double)D.n : 0.0, f);
+        mlp_forward1(D.mlp, f, y);
+        act = 0;
+        for (uint32_t q = 1; q < 3; ++q)
+            if (y[q] < y[act]) act = q;
+        if (!(y[act] == y[act])) act = GGPU_ACTION_BU;
+        ps.bottom_up = act == GGPU_ACTION_BU;
+    } else {
+        // the GAP rule on pair counts: frontier pair-edges against unreached pair-edges
+        if (!ps.bottom_up && (double)t.pairs > (double)pairs_u / D.alpha) ps.bottom_up = 1;
+        else if (ps.bottom_up && t.size < ps.prev_size && (double)t.size <= (double)D.n / D.beta) ps.bottom_up = 0;
+        act = ps.bottom_up ? GGPU_ACTION_BU : GGPU_ACTION_TD_BINNED;
+    }
+    ps.prev_size = t.size;
+    return act;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
+    __shared__ uint32_t s_act;
+    uint64_t clk[4] = {0, 0, 0, 0};
+    if (blockIdx.x == 0 && threadIdx.x == 0) clk[0] = wall_clock64();
+    MsAcc acc;
+    ms_seed<W>(D, acc);
+    ms_flush(D, 0, acc);
+    if (!grid_barrier(D.bar, D.abort)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) clk[1] = wall_clock64();
+
+    MsPolicyState ps;
+    uint32_t counts[3] = {0, 0, 0};
+    uint32_t prev_action = GGPU_ACTION_TD_BINNED;
+    uint32_t base = 0, level = 0;
+    const unsigned long long total_pairs = (unsigned long long)D.k * D.m;
+    uint64_t t_prev = 0;
+    for (;; ++level) {
+        const MsAcc t = ms_totals(D, level & 1);
+        if (threadIdx.x == 0) {
+            ps.cum_pairs += t.pairs;
+            const uint64_t pairs_u = total_pairs > ps.cum_pairs ? total_pairs - ps.cum_pairs : 0;
+            if (blockIdx.x == 0) {
+                const uint64_t now = wall_clock64();
+                if (level > 0 && level - 1 < D.trace_cap) {
+                    if (prev_action == GGPU_ACTION_BU) D.trace[level - 1].edges = t.examined;
+                    D.trace[level - 1].time_ms = (double)(now - t_prev) * D.tick_ms;
+                }
+                t_prev = now;
+            }
+            uint32_t act = 0xFFFFFFFFu;
+            if (t.size) {
+                act = ms_choose(D, level, t, pairs_u, ps);
+                counts[act]++;
+                if (blockIdx.x == 0 && level < D.trace_cap) {
+                    ggpu_level L{};
+                    L.level = level;
+                    L.action = act;
+                    L.frontier = t.size;
+                    L.edges = t.edges;
+                    L.frontier_edges = t.edges;
+                    L.unvisited_edges = (uint64_t)((double)pairs_u / (double)D.k);
+                    L.direction = act == GGPU_ACTION_BU ? 'B' : 'T';
+                    L.device = 'G';
+                    D.trace[level] = L;
+                }
+            }
+            s_act = act;
+        }
+        __syncthreads();
+        const uint32_t act = s_act;
+        __syncthreads();
+        if (act == 0xFFFFFFFFu) break;
+        prev_action = act;
+        if (level + 1 - base == D.C) {
+            // slot (level + 1) % C still holds level `base`: retire levels [base, level)
+            ms_materialize<W>(D, base, level - base, false);
+            if (!grid_barrier(D.bar, D.abort)) return;
+            base = level;
+        }
+        MsAcc a;
+        if (act == GGPU_ACTION_BU) {
+            ms_bottom_up<W>(D, level, a);
+        } else {
+            ms_td_push<W>(D, level, act == GGPU_ACTION_TD_BINNED);
+            if (!grid_barrier(D.bar, D.abort)) return;
+            ms_td_collect<W>(D, level, a);
+        }
+        ms_flush(D, (level + 1) & 1, a);
+        if (!grid_barrier(D.bar, D.abort)) return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) clk[2] = wall_clock64();
+    ms_materialize<W>(D, base, level - base, true);
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        clk[3] = wall_clock64();
+        D.out[0] = level;
+        D.out[2] = counts[0];
+        D.out[3] = counts[1];
+        D.out[4] = counts[2];
+        for (int i = 0; i < 3; ++i) D.out[6 + i] = (uint32_t)(clk[i + 1] - clk[i]);
+    }
+}
+
+// ------------------------------------------------------------------ host
+struct MsState {
+    DBuf<uint64_t> visit, next, F, fb;
+    DBuf<unsigned long long> part;
+    DBuf<uint32_t> dist, sources, ctl;
+    DBuf<uint4> heavy;
+    DBuf<ggpu_level> trace;
+    DBuf<uint8_t> actions;
+    uint32_t n_heavy = 0;
+    bool heavy_ready = false;
+    uint32_t W = 0, C = 0, k = 0;
+    uint32_t grid[5] = {0, 0, 0, 0, 0};
+    bool clean = false;  // visit / next / F / fb all zero
+    bool have = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~MsState() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+static void ms_state_free(void* p) { delete static_cast<MsState*>(p); }
+
+static MsState& ms_state(BfsEngine& e) {
+    if (!e.ms_state) {
+        e.ms_state = new MsState();
+        e.ms_free = ms_state_free;
+    }
+    return *static_cast<MsState*>(e.ms_state);
+}
+
+static const void* ms_kernel(uint32_t W) {
+    switch (W) {
+        case 1: return (const void*)k_msbfs<1>;
+        case 2: return (const void*)k_msbfs<2>;
+        case 3: return (const void*)k_msbfs<3>;
+        default: return (const void*)k_msbfs<4>;
+    }
+}
+
+static uint32_t ms_grid(ggpu_ctx* ctx, MsState& S, uint32_t W) {
+    if (!S.grid[W]) {
+        int per_cu = 0;
+        GGPU_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ms_kernel(W), kMsThreads, 0));
+        if (per_cu < 1) throw ArgError("batched BFS kernel does not fit on a compute unit", GGPU_EDEVICE);
+        S.grid[W] = (uint32_t)ctx->cus;  // one block per CU: the barrier cost grows with the block count
+    }
+    return S.grid[W];
+}
+
+}  // namespace ggpu
+
+using namespace ggpu;
+
+ggpu::BfsEngine* ggpu_bfs_engine(ggpu_bfs* b);
+
+extern "C" {
+
+int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_bfs_opts* opt,
+                   ggpu_msbfs_result* res, ggpu_level* trace, size_t trace_cap, size_t* trace_len) {
+    BfsEngine* ep = ggpu_bfs_engine(b);
+    if (!ep) return GGPU_EINVAL;
+    BfsEngine& e = *ep;
+    ggpu_ctx* ctx = e.ctx;
+    ggpu_bfs_opts o;
+    ggpu_bfs_opts_default(&o);
+    if (opt) o = *opt;
+    return guarded(ctx, [&] {
+        if (!sources) throw ArgError("sources is null");
+        if (k == 0 || k > 256) throw ArgError("batched BFS takes 1 to 256 sources");
+        for (uint32_t i = 0; i < k; ++i)
+            if (sources[i] >= e.n) throw ArgError("source out of range");
+        if (o.policy > GGPU_POLICY_SCRIPTED) throw ArgError("unknown policy");
+        if (o.policy == GGPU_POLICY_SCRIPTED && (!o.actions || !o.n_actions))
+            throw ArgError("scripted policy needs actions");
+        if (!(o.alpha > 0) || !(o.beta > 0)) throw ArgError("alpha and beta must be positive");
+        GGPU_HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        MsState& S = ms_state(e);
+        const uint32_t W = (k + 63) / 64;
+        const uint32_t n = e.n;
+        const uint32_t nb = (n + 63) / 64;
+        const size_t words = (size_t)n * W;
+        if (!S.ev[0]) {
+            GGPU_HIP_TRY(hipEventCreate(&S.ev[0]));
+            GGPU_HIP_TRY(hipEventCreate(&S.ev[1]));
+        }
+        if (!S.heavy_ready) {
+            std::vector<uint4> items;
+            for (uint32_t v = 0; v < n; ++v) {
+                const uint32_t dg = e.host.deg(v);
+                if (dg < kMsHeavy) continue;
+                for (uint32_t x = e.host.rp[v]; x < e.host.rp[v + 1]; x += kMsChunk)
+                    items.push_back(make_uint4(v, x, std::min(x + kMsChunk, e.host.rp[v + 1]), dg));
+            }
+            S.n_heavy = (uint32_t)items.size();
+            S.heavy.alloc(items.size() + 1);
+            if (!items.empty())
+                GGPU_HIP_TRY(hipMemcpy(S.heavy.p, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice));
+            S.heavy_ready = true;
+        }
+        // level slots: kMsSlots, fewer if device memory is short (at least 2)
+        if (S.W != W) {
+            S.visit.release();
+            S.next.release();
+            S.F.release();
+            S.fb.release();
+            size_t free_b = 0, total_b = 0;
+            GGPU_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+            const size_t slot_bytes = words * 8 + (size_t)nb * 8;
+            const size_t dist_bytes = (size_t)256 * n * 4;
+            const size_t avail = free_b > dist_bytes ? (free_b - dist_bytes) / 2 : 0;
+            const uint32_t C = (uint32_t)std::min<size_t>(kMsSlots, std::max<size_t>(2, avail / std::max<size_t>(slot_bytes, 1)));
+            S.visit.alloc(words);
+            S.next.alloc(words);
+            S.F.alloc((size_t)C * words);
+            S.fb.alloc((size_t)C * nb);
+            S.W = W;
+            S.C = C;
+            S.clean = false;
+        }
+        if (!S.clean) {
+            GGPU_HIP_TRY(hipMemsetAsync(S.visit.p, 0, words * 8, s));
+            GGPU_HIP_TRY(hipMemsetAsync(S.next.p, 0, words * 8, s));
+            GGPU_HIP_TRY(hipMemsetAsync(S.F.p, 0, (size_t)S.C * words * 8, s));
+            GGPU_HIP_TRY(hipMemsetAsync(S.fb.p, 0, (size_t)S.C * nb * 8, s));
+            S.clean = true;
+        }
+        const uint32_t G = ms_grid(ctx, S, W);
+        S.part.alloc((size_t)2 * G * kPartWords);
+        S.dist.alloc((size_t)k * n);
+        S.k = k;
+        S.sources.alloc(k);
+        S.ctl.alloc(kBarWords + 16);
+        const uint32_t tcap = (uint32_t)std::min<uint64_t>((uint64_t)n + 1, kMsTraceCap);
+        S.trace.alloc(tcap);
+        if (o.policy == GGPU_POLICY_SCRIPTED) {
+            S.actions.alloc(o.n_actions);
+            GGPU_HIP_TRY(hipMemcpyAsync(S.actions.p, o.actions, o.n_actions, hipMemcpyHostToDevice, s));
+        }
+        const bool learned = o.policy == GGPU_POLICY_LEARNED && ctx->policy_set;
+        if (learned)
+            GGPU_HIP_TRY(hipMemcpyAsync(ctx->policy_dev, ctx->policy_host.data(), GGPU_MLP_PARAMS * sizeof(float),
+                                        hipMemcpyHostToDevice, s));
+        GGPU_HIP_TRY(hipMemcpyAsync(S.sources.p, sources, 4ull * k, hipMemcpyHostToDevice, s));
+        GGPU_HIP_TRY(hipMemsetAsync(S.ctl.p, 0, (kBarWords + 16) * sizeof(uint32_t), s));
+
+        MsDev D{};
+        D.rp = e.rp.p;
+        D.ci = e.ci.p;
+        D.visit = S.visit.p;
+        D.next = S.next.p;
+        D.F = S.F.p;
+        D.fb = S.fb.p;
+        D.part = S.part.p;
+        D.dist = S.dist.p;
+        D.sources = S.sources.p;
+        D.heavy = S.heavy.p;
+        D.bar = S.ctl.p;
+        D.abort = S.ctl.p + kBarWords;
+        D.out = S.ctl.p + kBarWords + 4;
+        D.trace = S.trace.p;
+        D.actions = o.policy == GGPU_POLICY_SCRIPTED ? S.actions.p : nullptr;
+        D.mlp = learned ? ctx->policy_dev : nullptr;
+        D.alpha = o.alpha;
+        D.beta = o.beta;
+        int clock_khz = 0;
+        GGPU_HIP_TRY(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device));
+        D.tick_ms = clock_khz > 0 ? 1.0 / clock_khz : 0.0;
+        D.m = e.m;
+        D.valid_last = (k % 64) ? ((1ull << (k % 64)) - 1) : ~0ull;
+        D.n = n;
+        D.k = k;
+        D.nb = nb;
+        D.n_heavy = S.n_heavy;
+        D.C = S.C;
+        D.trace_cap = tcap;
+        D.n_actions = o.n_actions;
+        D.policy = o.policy;
+
+        void* args[] = {&D};
+        S.have = false;
+        S.clean = false;  // until the launch is known to have finished
+        GGPU_HIP_TRY(hipEventRecord(S.ev[0], s));
+        GGPU_HIP_TRY(hipLaunchCooperativeKernel(ms_kernel(W), dim3(G), dim3(kMsThreads), args, 0, s));
+        GGPU_HIP_TRY(hipEventRecord(S.ev[1], s));
+        uint32_t ctl[16];
+        GGPU_HIP_TRY(hipMemcpyAsync(ctl, S.ctl.p + kBarWords, sizeof ctl, hipMemcpyDeviceToHost, s));
+        GGPU_HIP_TRY(hipStreamSynchronize(s));
+        if (ctl[0]) throw ArgError("batched BFS: grid barrier timed out", GGPU_ETIMEOUT);
+        S.clean = true;
+        float ms = 0;
+        GGPU_HIP_TRY(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
+        const uint32_t levels = ctl[4];
+        e.last_gpu_ms = ms;
+        e.last_levels = levels;
+        if (res) {
+            res->levels = levels;
+            res->pad = 0;
+            res->gpu_ms = ms;
+            res->level_actions[0] = ctl[6];
+            res->level_actions[1] = ctl[7];
+            res->level_actions[2] = ctl[8];
+            res->setup_ms = ctl[10] * D.tick_ms;
+            res->levels_ms = ctl[11] * D.tick_ms;
+            res->finish_ms = ctl[12] * D.tick_ms;
+        }
+        if (trace_len) *trace_len = levels;
+        if (trace && trace_cap) {
+            const size_t cnt = std::min<size_t>({(size_t)levels, trace_cap, (size_t)tcap});
+            if (cnt) GGPU_HIP_TRY(hipMemcpy(trace, S.trace.p, cnt * sizeof(ggpu_level), hipMemcpyDeviceToHost));
+        }
+        S.have = true;
+    });
+}
+
+int ggpu_msbfs_fetch_dist(ggpu_bfs* b, uint32_t* out) {
+    BfsEngine* ep = ggpu_bfs_engine(b);
+    if (!ep) return GGPU_EINVAL;
+    return guarded(ep->ctx, [&] {
+        if (!out) throw ArgError("out is null");
+        if (!ep->ms_state || !static_cast<MsState*>(ep->ms_state)->have)
+            throw ArgError("no batched BFS result yet", GGPU_ESTATE);
+        MsState& S = *static_cast<MsState*>(ep->ms_state);
+        GGPU_HIP_TRY(hipSetDevice(ep->ctx->device));
+        GGPU_HIP_TRY(hipMemcpyAsync(out, S.dist.p, 4ull * S.k * ep->n, hipMemcpyDeviceToHost, ep->ctx->stream));
+        GGPU_HIP_TRY(hipStreamSynchronize(ep->ctx->stream));
+    });
+}
+
+}  // extern "C"

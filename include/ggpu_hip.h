@@ -104,6 +104,9 @@ typedef struct ggpu_msbfs_result {
     uint32_t pad;
     double gpu_ms;            /* device time of the whole batch (events) */
     uint64_t level_actions[3];/* levels run per action */
+    double setup_ms;          /* device clock: launch start -> level 0 ready */
+    double levels_ms;         /* device clock: the level loop */
+    double finish_ms;         /* device clock: depths written, state cleared (block 0's view) */
 } ggpu_msbfs_result;
 /* opt->policy picks the per-level action (HEURISTIC: GAP rule on batch counts;
  * LEARNED: the MLP; SCRIPTED: opt->actions). */

@@ -1,0 +1,120 @@
+"""Batched multi-source BFS parity: every source's depths equal the oracle's
+single-source BFS from it, bit for bit, whatever the batch size, the
+duplicates in the batch, the per-level action sequence and the state left
+by earlier batches on the same object.
+
+The batched BFS has no counterpart in the reference; its oracle is the
+reference's serial BFS (src/cpu/bfs_cpu.cpp:10-29, restated in
+oracle/oracle.c), run once per source.
+"""
+import numpy as np
+import pytest
+
+import graph_analytics_applesilicon_amd as G
+from oracle import pyoracle as O
+from tests.conftest import SUITE, hub, sample_sources
+
+pytestmark = pytest.mark.gpu
+
+_GRAPHS = {}
+
+
+def _graph(name):
+    if name not in _GRAPHS:
+        g = G.Graph.named(name).symmetrize()
+        _GRAPHS[name] = (g, G.GpuBfs(g))
+    return _GRAPHS[name]
+
+
+def _batch(g, k, seed=7):
+    """k sources: the sample sources and the hub first, then seeded random ones
+    (with repeats when k exceeds n or by chance)."""
+    base = list(dict.fromkeys(sample_sources(g.n) + [hub(g)]))
+    rng = np.random.default_rng(seed)
+    extra = rng.integers(0, g.n, size=max(0, k - len(base))).tolist()
+    return np.array((base + extra)[:k], dtype=np.uint32)
+
+
+def _check(g, bfs, src, opts, tag):
+    dist, res, trace = bfs.run_batch(src, opts, trace=True)
+    assert dist.shape == (len(src), g.n)
+    depth = 0
+    for i, s in enumerate(src):
+        ref = O.bfs_serial(g.row_ptr, g.col_idx, int(s))
+        assert np.array_equal(dist[i], ref), f"{tag}: source #{i} ({s})"
+        fin = ref[ref != O.INF]
+        depth = max(depth, int(fin.max()) + 1)
+    assert res.levels == depth, tag
+    assert sum(res.level_actions) == depth, tag
+    assert trace[0].frontier == len(set(int(s) for s in src)), tag
+    return res, trace
+
+
+MODES = {
+    "heuristic": dict(),
+    "all-bottom-up": dict(alpha=1e12, beta=1e12),
+    "all-top-down": dict(alpha=1e-9),
+}
+
+
+@pytest.mark.parametrize("name", SUITE)
+@pytest.mark.parametrize("k", [1, 5, 64, 100, 256])
+def test_batched_matches_serial_per_source(name, k):
+    g, bfs = _graph(name)
+    src = _batch(g, k)
+    for mode, kw in MODES.items():
+        _check(g, bfs, src, G.bfs_options(**kw), f"{name} k={k} {mode}")
+
+
+@pytest.mark.parametrize("name", ["rmat12", "grid100x100", "path1000", "star100k"])
+def test_batched_scripted_actions(name):
+    """Any per-level mix of the three actions gives the same depths."""
+    g, bfs = _graph(name)
+    rng = np.random.default_rng(11)
+    for k in (3, 64, 130):
+        src = _batch(g, k, seed=k)
+        for trial in range(4):
+            actions = rng.integers(0, 3, size=int(rng.integers(1, 9))).tolist()
+            res, trace = _check(g, bfs, src, G.bfs_options(policy=G.POLICY_SCRIPTED, actions=actions),
+                                f"{name} k={k} actions={actions}")
+            for L in trace[: len(trace)]:
+                assert L.action == actions[L.level % len(actions)]
+
+
+def test_batched_all_same_source_and_state_reuse():
+    """Duplicates, shrinking and growing word counts on one object."""
+    g, bfs = _graph("rmat12")
+    ref = O.bfs_serial(g.row_ptr, g.col_idx, 17)
+    for k in (256, 1, 64, 65, 2):
+        dist, res, _ = bfs.run_batch(np.full(k, 17, dtype=np.uint32))
+        assert all(np.array_equal(dist[i], ref) for i in range(k))
+    src = _batch(g, 200)
+    d1, _, _ = bfs.run_batch(src)
+    d2, _, _ = bfs.run_batch(src)
+    assert np.array_equal(d1, d2)
+
+
+def test_batched_larger_graph():
+    """A 65,536-vertex RMAT graph (rmat(16, 16, .57, .19, .19), permuted)."""
+    g = G.Graph.named("rmat16_c1")
+    bfs = G.GpuBfs(g)
+    src = np.array(g.pick_sources(64, 1), dtype=np.uint32)
+    for mode, kw in MODES.items():
+        _check(g, bfs, src, G.bfs_options(**kw), f"rmat16_c1 {mode}")
+
+
+def test_batched_errors():
+    g, _ = _graph("rmat12")
+    fresh = G.GpuBfs(g)
+    with pytest.raises(ValueError):
+        fresh.run_batch(np.array([], dtype=np.uint32))
+    with pytest.raises(ValueError):
+        fresh.run_batch(np.zeros(257, dtype=np.uint32))
+    with pytest.raises(ValueError):
+        fresh.run_batch(np.array([g.n], dtype=np.uint32))
+    with pytest.raises(ValueError):
+        fresh.run_batch(np.array([0], dtype=np.uint32), G.bfs_options(policy=G.POLICY_SCRIPTED))
+    # nothing has run on `fresh`: fetching is a state error
+    out = np.zeros(g.n, dtype=np.uint32)
+    rc = fresh.ctx.lib.ggpu_msbfs_fetch_dist(fresh._h, out.ctypes.data_as(G.C.POINTER(G.C.c_uint32)))
+    assert rc == 5
