@@ -43,6 +43,7 @@ PR_ORDER_GIVEN, PR_ORDER_DEGREE = 0, 1
 PR_VARIANTS = {"gpu_pull_scalar": PR_SCALAR, "gpu_pull_vector": PR_VECTOR,
                "gpu_pull_adaptive": PR_ADAPTIVE, "gpu_pull_gs": PR_GS}
 MLP_PARAMS = 1347
+POLICY_SINGLE, POLICY_BATCHED = 0, 1
 
 
 # ------------------------------------------------------------------ structs
@@ -193,9 +194,9 @@ def _optional_hip_symbols():
         "ggpu_pr_autotune": ([vp, C.c_int, C.POINTER(PrOpts)], C.c_int),
         "ggpu_ppr_run": ([vp, _u32p, C.c_uint32, C.POINTER(PrOpts), C.POINTER(PrResult)], C.c_int),
         "ggpu_ppr_fetch": ([vp, fp], C.c_int),
-        "ggpu_policy_set": ([vp, fp, C.c_uint32], C.c_int),
-        "ggpu_policy_clear": ([vp], C.c_int),
-        "ggpu_policy_get": ([vp, fp, C.c_uint32, C.POINTER(C.c_int)], C.c_int),
+        "ggpu_policy_set": ([vp, C.c_int, fp, C.c_uint32], C.c_int),
+        "ggpu_policy_clear": ([vp, C.c_int], C.c_int),
+        "ggpu_policy_get": ([vp, C.c_int, fp, C.c_uint32, C.POINTER(C.c_int)], C.c_int),
         "ggpu_policy_features": ([C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, fp], None),
         "ggpu_mlp_forward": ([vp, fp, fp, C.c_uint64, fp], C.c_int),
         "ggpu_mlp_loss_grad": ([vp, fp, fp, fp, C.POINTER(C.c_uint8), C.c_uint64, fp, fp], C.c_int),
@@ -203,6 +204,8 @@ def _optional_hip_symbols():
                            C.c_int),
         "ggpu_bfs_record_levels": ([vp, C.c_uint32, C.POINTER(BfsOpts), fp, fp, C.POINTER(Level), C.c_size_t,
                                     C.POINTER(C.c_size_t)], C.c_int),
+        "ggpu_msbfs_record_levels": ([vp, _u32p, C.c_uint32, C.POINTER(BfsOpts), fp, fp, C.POINTER(Level),
+                                      C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
     }
 
 
@@ -515,6 +518,27 @@ class GpuBfs:
         levels = [tr[i] for i in range(min(ln.value, cap))] if trace else []
         return dist, res, levels
 
+    # policy training data
+    def _record(self, fn, args, opts, cap):
+        o = opts or bfs_options()
+        feat = np.zeros((cap, 5), dtype=np.float32)
+        times = np.zeros((cap, 3), dtype=np.float32)
+        tr = (Level * cap)()
+        ln = C.c_size_t()
+        self.ctx.check(fn(self._h, *args, C.byref(o), _ptr(feat, C.c_float), _ptr(times, C.c_float), tr, cap,
+                          C.byref(ln)))
+        k = min(ln.value, cap)
+        return feat[:k], times[:k], [tr[i] for i in range(k)]
+
+    def record_levels(self, source: int, opts: Optional[BfsOpts] = None, cap: int = 4096):
+        """gpu_do from `source` once per action: (features[L,5], times_ms[L,3], trace)."""
+        return self._record(self.ctx.lib.ggpu_bfs_record_levels, (int(source),), opts, cap)
+
+    def record_batch_levels(self, sources, opts: Optional[BfsOpts] = None, cap: int = 4096):
+        """Batched BFS once per action: (features[L,5], times_ms[L,3], trace)."""
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        return self._record(self.ctx.lib.ggpu_msbfs_record_levels, (_ptr(src), len(src)), opts, cap)
+
 
 def pr_options(**kw) -> PrOpts:
     o = PrOpts()
@@ -596,22 +620,95 @@ def policy_features(n_f, m_f, m_u, level, mean_degree) -> np.ndarray:
     return f
 
 
-def policy_set(params, ctx: Optional[Context] = None):
+def policy_set(params, ctx: Optional[Context] = None, slot: int = POLICY_BATCHED):
+    """Installs MLP parameters for single-source (POLICY_SINGLE) or batched
+    (POLICY_BATCHED) traversals run with policy=POLICY_LEARNED."""
     c = ctx or default_context()
     p = np.ascontiguousarray(params, dtype=np.float32)
-    c.check(c.lib.ggpu_policy_set(c._h, _ptr(p, C.c_float), len(p)))
+    c.check(c.lib.ggpu_policy_set(c._h, slot, _ptr(p, C.c_float), len(p)))
 
 
-def policy_clear(ctx: Optional[Context] = None):
+def policy_clear(ctx: Optional[Context] = None, slot: int = POLICY_BATCHED):
     c = ctx or default_context()
-    c.check(c.lib.ggpu_policy_clear(c._h))
+    c.check(c.lib.ggpu_policy_clear(c._h, slot))
 
 
-POLICY_FILE = os.path.join(_PKG, "policy", "policy_weights.f32")
+def policy_get(ctx: Optional[Context] = None, slot: int = POLICY_BATCHED) -> Optional[np.ndarray]:
+    c = ctx or default_context()
+    p = np.zeros(MLP_PARAMS, dtype=np.float32)
+    is_set = C.c_int()
+    c.check(c.lib.ggpu_policy_get(c._h, slot, _ptr(p, C.c_float), MLP_PARAMS, C.byref(is_set)))
+    return p if is_set.value else None
 
 
-def load_policy_file(path: str = POLICY_FILE) -> Optional[np.ndarray]:
+def mlp_init(seed: int = 0) -> np.ndarray:
+    """He-initialised parameters (biases zero) in the layout W1 b1 W2 b2 W3 b3."""
+    rng = np.random.default_rng(seed)
+    parts = [rng.normal(0, np.sqrt(2 / 5), 32 * 5), np.zeros(32), rng.normal(0, np.sqrt(2 / 32), 32 * 32),
+             np.zeros(32), rng.normal(0, np.sqrt(2 / 32), 3 * 32), np.zeros(3)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def mlp_forward(params, x, ctx: Optional[Context] = None) -> np.ndarray:
+    c = ctx or default_context()
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 5)
+    y = np.zeros((len(xx), 3), dtype=np.float32)
+    c.check(c.lib.ggpu_mlp_forward(c._h, _ptr(p, C.c_float), _ptr(xx, C.c_float), len(xx), _ptr(y, C.c_float)))
+    return y
+
+
+def _mlp_data(x, t, mask):
+    xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 5)
+    tt = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 3)
+    mk = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1, 3)
+    return xx, tt, mk
+
+
+def mlp_loss_grad(params, x, t, mask=None, ctx: Optional[Context] = None):
+    """GPU loss and gradient of the masked mean squared error."""
+    c = ctx or default_context()
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx, tt, mk = _mlp_data(x, t, mask)
+    g = np.zeros(MLP_PARAMS, dtype=np.float32)
+    loss = C.c_float()
+    c.check(c.lib.ggpu_mlp_loss_grad(c._h, _ptr(p, C.c_float), _ptr(xx, C.c_float), _ptr(tt, C.c_float),
+                                     None if mk is None else _ptr(mk, C.c_uint8), len(xx), _ptr(g, C.c_float),
+                                     C.byref(loss)))
+    return float(loss.value), g
+
+
+def mlp_train(params, x, t, mask=None, epochs: int = 1000, lr: float = 1e-2, ctx: Optional[Context] = None):
+    """Full-batch Adam on the GPU; returns (params, final loss)."""
+    c = ctx or default_context()
+    p = np.array(params, dtype=np.float32)
+    xx, tt, mk = _mlp_data(x, t, mask)
+    loss = C.c_float()
+    c.check(c.lib.ggpu_mlp_train(c._h, _ptr(p, C.c_float), _ptr(xx, C.c_float), _ptr(tt, C.c_float),
+                                 None if mk is None else _ptr(mk, C.c_uint8), len(xx), epochs, lr,
+                                 C.byref(loss)))
+    return p, float(loss.value)
+
+
+POLICY_DIR = os.path.join(_PKG, "policy")
+POLICY_FILES = {POLICY_SINGLE: os.path.join(POLICY_DIR, "single_source.f32"),
+                POLICY_BATCHED: os.path.join(POLICY_DIR, "batched.f32")}
+
+
+def load_policy_file(slot: int = POLICY_BATCHED, path: Optional[str] = None) -> Optional[np.ndarray]:
+    path = path or POLICY_FILES[slot]
     if not os.path.exists(path):
         return None
     p = np.fromfile(path, dtype=np.float32)
     return p if len(p) == MLP_PARAMS else None
+
+
+def install_policies(ctx: Optional[Context] = None) -> dict:
+    """Sets every slot whose trained parameter file exists; returns {slot: installed}."""
+    done = {}
+    for slot in (POLICY_SINGLE, POLICY_BATCHED):
+        p = load_policy_file(slot)
+        if p is not None:
+            policy_set(p, ctx, slot)
+        done[slot] = p is not None
+    return done

@@ -174,7 +174,7 @@ void BfsEngine::host_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, std
     BfsDev d = dev();
     LoopState st = loop_start(src, variant, o);
     const PolicyView pv =
-        policy_view(variant, o, ctx->policy_set ? ctx->policy_host.data() : nullptr);
+        policy_view(variant, o, ctx->policy_set[GGPU_POLICY_SINGLE] ? ctx->policy_host[GGPU_POLICY_SINGLE].data() : nullptr);
     const uint32_t src_bin = bin_of(host.deg(src), o.small_max, o.large_min);
     last_levels = 0;
     last_gpu_ms = 0;

@@ -143,10 +143,11 @@ struct ggpu_ctx {
     std::string name;
     std::string err;
     uint64_t hbm = 0;
-    // learned policy parameters (device copy, GGPU_MLP_PARAMS floats)
-    bool policy_set = false;
-    std::vector<float> policy_host;
-    float* policy_dev = nullptr;
+    // learned policy parameters per slot (GGPU_POLICY_SINGLE / _BATCHED),
+    // host and device copies of GGPU_MLP_PARAMS floats
+    bool policy_set[2] = {false, false};
+    std::vector<float> policy_host[2];
+    float* policy_dev[2] = {nullptr, nullptr};
     // pinned host staging for per-level control readback
     void* pinned = nullptr;
     size_t pinned_bytes = 0;

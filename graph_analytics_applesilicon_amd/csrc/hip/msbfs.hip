@@ -56,6 +56,7 @@ struct MsDev {
     uint32_t* abort;
     uint32_t* out;                 // {levels, 0, actions[3], 0, clock at start, level 0, loop end, finish}
     ggpu_level* trace;             // trace_cap entries
+    float* feat;                   // trace_cap x 5 policy features (as the policy sees them)
     const uint8_t* actions;        // scripted policy
     const float* mlp;              // learned policy parameters or null
     double alpha, beta;
@@ -535,19 +536,25 @@ struct MsPolicyState {
 };
 
 // Out of line: the MLP's hidden layers stay out of the traversal's registers.
-__device__ __noinline__ uint32_t ms_choose(const MsDev& D, uint32_t level, const MsAcc& t, uint64_t pairs_u,
-                                           MsPolicyState& ps) {
-    uint32_t act;
+__device__ __forceinline__ void ms_features(const MsDev& D, uint32_t level, const MsAcc& t, uint64_t pairs_u,
+                                            float* f) {
+    // frontier vertices, frontier and unreached edge counts per source (pair-edges / k)
     const double kk = (double)D.k;
+    policy_features(t.size, (uint64_t)((double)t.pairs / kk), (uint64_t)((double)pairs_u / kk), level,
+                    D.n ? (double)D.m / (double)D.n : 0.0, f);
+}
+
+__device__ __noinline__ uint32_t ms_choose(const MsDev& D, const float* mlp, uint32_t level, const MsAcc& t,
+                                           uint64_t pairs_u, MsPolicyState& ps) {
+    uint32_t act;
     if (D.policy == GGPU_POLICY_SCRIPTED) {
         act = D.actions[level % D.n_actions];
         if (act > GGPU_ACTION_BU) act = GGPU_ACTION_BU;
         ps.bottom_up = act == GGPU_ACTION_BU;
     } else if (D.policy == GGPU_POLICY_LEARNED && D.mlp) {
         float f[5], y[3];
-        policy_features(t.size, (uint64_t)((double)t.pairs / kk), (uint64_t)((double)pairs_u / kk), level,
-                        D.n ? (double)D.m / (double)D.n : 0.0, f);
-        mlp_forward1(D.mlp, f, y);
+        ms_features(D, level, t, pairs_u, f);
+        mlp_forward1(mlp, f, y);
         act = 0;
         for (uint32_t q = 1; q < 3; ++q)
             if (y[q] < y[act]) act = q;
@@ -566,6 +573,9 @@ __device__ __noinline__ uint32_t ms_choose(const MsDev& D, uint32_t level, const
 template <int W>
 __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
     __shared__ uint32_t s_act;
+    __shared__ float s_mlp[GGPU_MLP_PARAMS];  // the learned policy, read at every level boundary
+    if (D.mlp)
+        for (uint32_t i = threadIdx.x; i < GGPU_MLP_PARAMS; i += blockDim.x) s_mlp[i] = D.mlp[i];
     uint64_t clk[4] = {0, 0, 0, 0};
     if (blockIdx.x == 0 && threadIdx.x == 0) clk[0] = wall_clock64();
     MsAcc acc;
@@ -595,7 +605,7 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
             }
             uint32_t act = 0xFFFFFFFFu;
             if (t.size) {
-                act = ms_choose(D, level, t, pairs_u, ps);
+                act = ms_choose(D, s_mlp, level, t, pairs_u, ps);
                 counts[act]++;
                 if (blockIdx.x == 0 && level < D.trace_cap) {
                     ggpu_level L{};
@@ -608,6 +618,7 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
                     L.direction = act == GGPU_ACTION_BU ? 'B' : 'T';
                     L.device = 'G';
                     D.trace[level] = L;
+                    if (D.feat) ms_features(D, level, t, pairs_u, D.feat + (size_t)level * 5);
                 }
             }
             s_act = act;
@@ -654,6 +665,7 @@ struct MsState {
     DBuf<uint32_t> dist, sources, ctl;
     DBuf<uint4> heavy;
     DBuf<ggpu_level> trace;
+    DBuf<float> feat;
     DBuf<uint8_t> actions;
     uint32_t n_heavy = 0;
     bool heavy_ready = false;
@@ -783,14 +795,12 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         S.ctl.alloc(kBarWords + 16);
         const uint32_t tcap = (uint32_t)std::min<uint64_t>((uint64_t)n + 1, kMsTraceCap);
         S.trace.alloc(tcap);
+        S.feat.alloc((size_t)tcap * 5);
         if (o.policy == GGPU_POLICY_SCRIPTED) {
             S.actions.alloc(o.n_actions);
             GGPU_HIP_TRY(hipMemcpyAsync(S.actions.p, o.actions, o.n_actions, hipMemcpyHostToDevice, s));
         }
-        const bool learned = o.policy == GGPU_POLICY_LEARNED && ctx->policy_set;
-        if (learned)
-            GGPU_HIP_TRY(hipMemcpyAsync(ctx->policy_dev, ctx->policy_host.data(), GGPU_MLP_PARAMS * sizeof(float),
-                                        hipMemcpyHostToDevice, s));
+        const bool learned = o.policy == GGPU_POLICY_LEARNED && ctx->policy_set[GGPU_POLICY_BATCHED];
         GGPU_HIP_TRY(hipMemcpyAsync(S.sources.p, sources, 4ull * k, hipMemcpyHostToDevice, s));
         GGPU_HIP_TRY(hipMemsetAsync(S.ctl.p, 0, (kBarWords + 16) * sizeof(uint32_t), s));
 
@@ -809,8 +819,9 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         D.abort = S.ctl.p + kBarWords;
         D.out = S.ctl.p + kBarWords + 4;
         D.trace = S.trace.p;
+        D.feat = S.feat.p;
         D.actions = o.policy == GGPU_POLICY_SCRIPTED ? S.actions.p : nullptr;
-        D.mlp = learned ? ctx->policy_dev : nullptr;
+        D.mlp = learned ? ctx->policy_dev[GGPU_POLICY_BATCHED] : nullptr;
         D.alpha = o.alpha;
         D.beta = o.beta;
         int clock_khz = 0;
@@ -860,6 +871,47 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             if (cnt) GGPU_HIP_TRY(hipMemcpy(trace, S.trace.p, cnt * sizeof(ggpu_level), hipMemcpyDeviceToHost));
         }
         S.have = true;
+    });
+}
+
+int ggpu_msbfs_record_levels(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_bfs_opts* opt,
+                             float* features, float* times_ms, ggpu_level* trace, size_t cap, size_t* len) {
+    BfsEngine* ep = ggpu_bfs_engine(b);
+    if (!ep) return GGPU_EINVAL;
+    ggpu_ctx* ctx = ep->ctx;
+    return guarded(ctx, [&] {
+        ggpu_bfs_opts o;
+        ggpu_bfs_opts_default(&o);
+        if (opt) o = *opt;
+        o.policy = GGPU_POLICY_SCRIPTED;
+        o.n_actions = 1;
+        std::vector<ggpu_level> runs[3];
+        std::vector<float> feat;
+        size_t levels = 0;
+        for (uint8_t a = 0; a < 3; ++a) {
+            o.actions = &a;
+            runs[a].resize(kMsTraceCap);
+            size_t ln = 0;
+            const int rc = ggpu_msbfs_run(b, sources, k, &o, nullptr, runs[a].data(), runs[a].size(), &ln);
+            if (rc != GGPU_OK) throw ArgError(ctx->err, rc);
+            if (a == 0) {
+                levels = ln;
+                MsState& S = *static_cast<MsState*>(ep->ms_state);
+                const size_t kept = std::min<size_t>(levels, S.trace.count);
+                feat.resize(kept * 5);
+                if (kept) GGPU_HIP_TRY(hipMemcpy(feat.data(), S.feat.p, kept * 5 * sizeof(float), hipMemcpyDeviceToHost));
+            } else if (ln != levels) {
+                throw ArgError("level counts differ between actions", GGPU_ESTATE);
+            }
+        }
+        const size_t cnt = std::min({levels, cap, feat.size() / 5});
+        for (size_t l = 0; l < cnt; ++l) {
+            if (features) std::memcpy(features + l * 5, feat.data() + l * 5, 5 * sizeof(float));
+            if (times_ms)
+                for (int a = 0; a < 3; ++a) times_ms[l * 3 + a] = (float)runs[a][l].time_ms;
+            if (trace) trace[l] = runs[0][l];
+        }
+        if (len) *len = levels;
     });
 }
 

@@ -47,7 +47,7 @@ int ggpu_ctx_create(int device, ggpu_ctx** out) {
         c->cus = prop.multiProcessorCount;
         c->hbm = prop.totalGlobalMem;
         GGPU_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        GGPU_HIP_TRY(hipMalloc((void**)&c->policy_dev, GGPU_MLP_PARAMS * sizeof(float)));
+        for (auto& p : c->policy_dev) GGPU_HIP_TRY(hipMalloc((void**)&p, GGPU_MLP_PARAMS * sizeof(float)));
         ctx_pinned(c, 4096);
     });
     if (rc != GGPU_OK) {
@@ -67,7 +67,8 @@ void ggpu_ctx_destroy(ggpu_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
     }
-    if (ctx->policy_dev) (void)hipFree(ctx->policy_dev);
+    for (auto p : ctx->policy_dev)
+        if (p) (void)hipFree(p);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     delete ctx;
 }

@@ -188,11 +188,17 @@ int ggpu_ppr_fetch(ggpu_pr* p, float* out);
  * No reference counterpart. MLP 5 -> 32 -> 32 -> 3 (ReLU), FP32; 1,347
  * parameters laid out W1[32][5] b1[32] W2[32][32] b2[32] W3[3][32] b3[3].
  * Inputs are the per-level features of ggpu_policy_features; outputs are
- * predicted log-times of the three actions; the policy takes the argmin. */
+ * predicted log-times (ln ms) of the actions TD_EDGE, TD_BINNED, BU; the
+ * policy takes the argmin. A context holds one parameter set per slot:
+ * single-source traversals (gpu_do) and batched traversals have different
+ * costs per action. With no set for a slot, LEARNED falls back to the
+ * heuristic. */
 #define GGPU_MLP_PARAMS 1347
-int ggpu_policy_set(ggpu_ctx* ctx, const float* params, uint32_t count);
-int ggpu_policy_clear(ggpu_ctx* ctx);
-int ggpu_policy_get(ggpu_ctx* ctx, float* params, uint32_t count, int* is_set);
+#define GGPU_POLICY_SINGLE 0   /* gpu_do */
+#define GGPU_POLICY_BATCHED 1  /* ggpu_msbfs_run */
+int ggpu_policy_set(ggpu_ctx* ctx, int slot, const float* params, uint32_t count);
+int ggpu_policy_clear(ggpu_ctx* ctx, int slot);
+int ggpu_policy_get(ggpu_ctx* ctx, int slot, float* params, uint32_t count, int* is_set);
 /* features[5] of one level: log2(1+n_f)/32, log2(1+m_f)/32, log2(1+m_u)/32,
  * level/64, log2(1+mean degree)/16. */
 void ggpu_policy_features(uint64_t n_f, uint64_t m_f, uint64_t m_u, uint32_t level, double mean_degree,
@@ -205,12 +211,18 @@ int ggpu_mlp_loss_grad(ggpu_ctx* ctx, const float* params, const float* x, const
 /* GPU training: `epochs` full-batch Adam steps from params_inout. */
 int ggpu_mlp_train(ggpu_ctx* ctx, float* params_inout, const float* x, const float* t, const uint8_t* mask,
                    uint64_t count, uint32_t epochs, float lr, float* final_loss);
-/* Training data: one BFS from `source` whose every level is timed under all
- * three actions from the same state (the state is restored between them);
- * the traversal then follows the fastest. Writes per level the 5 features,
- * the 3 times (ms) and the trace record. *len = levels. */
+/* Training data. The frontier and the counts of a level do not depend on
+ * the actions taken before it, so the traversal is run once per action with
+ * that action at every level: level l's three times are the three actions'
+ * costs from the same state. Writes per level the 5 features (as the policy
+ * sees them), the 3 times (ms, features[l*5..], times_ms[l*3..]) and the
+ * trace record of the first run; *len = levels (at most cap are written).
+ * Single source (gpu_do's level loop, device time per level from events): */
 int ggpu_bfs_record_levels(ggpu_bfs* b, uint32_t source, const ggpu_bfs_opts* opt, float* features,
                            float* times_ms, ggpu_level* trace, size_t cap, size_t* len);
+/* Batched (one ggpu_msbfs_run per action; times from the device clock): */
+int ggpu_msbfs_record_levels(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_bfs_opts* opt,
+                             float* features, float* times_ms, ggpu_level* trace, size_t cap, size_t* len);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,172 @@
+"""Learned direction policy on the GPU.
+
+The MLP kernels are checked against the oracle's FP64 restatement (forward,
+loss and analytic gradient; the oracle's gradient is itself checked by finite
+differences in the CPU tests), one Adam step against its closed form, the
+level recorders against the traversals they time, the inference fused into
+the level boundary against the forward kernel's argmin, and depths stay
+bit-exact under arbitrary weights. No reference counterpart: the reference
+has only the heuristic rule (src/cpu/bfs_cpu.cpp:227-283).
+"""
+import numpy as np
+import pytest
+
+import graph_analytics_applesilicon_amd as G
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    p = G.mlp_init(seed)
+    return (p + rng.normal(0, 0.05, p.shape)).astype(np.float32)  # non-zero biases too
+
+
+def _data(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1, (n, 5)).astype(np.float32)
+    t = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    return x, t
+
+
+def test_forward_matches_oracle():
+    for seed in range(3):
+        p = _params(seed)
+        x, _ = _data(1000 + seed, seed)
+        y = G.mlp_forward(p, x)
+        ref = O.mlp_forward(p.astype(np.float64), x.astype(np.float64))
+        np.testing.assert_allclose(y, ref, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("n", [1, 7, 1000, 2500])
+@pytest.mark.parametrize("masked", [False, True])
+def test_loss_grad_matches_oracle(n, masked):
+    p = _params(n)
+    x, t = _data(n, n + 1)
+    mask = None
+    if masked:
+        mask = (np.random.default_rng(n).uniform(size=(n, 3)) < 0.6).astype(np.uint8)
+        mask[0, 0] = 1
+    loss, g = G.mlp_loss_grad(p, x, t, mask)
+    rl, rg = O.mlp_loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), mask)
+    assert abs(loss - rl) <= 1e-5 * max(1.0, abs(rl))
+    scale = max(np.abs(rg).max(), 1e-12)
+    np.testing.assert_allclose(g, rg, rtol=1e-4, atol=1e-5 * scale)
+
+
+def test_adam_first_step_closed_form():
+    """After one step Adam moves each parameter by lr * g / (|g| + 1e-8)."""
+    p = _params(5)
+    x, t = _data(300, 6)
+    _, rg = O.mlp_loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    lr = 1e-3
+    p1, _ = G.mlp_train(p, x, t, epochs=1, lr=lr)
+    want = p.astype(np.float64) - lr * rg / (np.abs(rg) + 1e-8)
+    sure = np.abs(rg) > 1e-4  # where FP32 and FP64 agree on the sign
+    np.testing.assert_allclose(p1[sure], want[sure], rtol=0, atol=2e-6)
+    assert np.all(np.abs(p1 - p) <= lr * 1.0001)
+
+
+def test_training_fits_a_learnable_target():
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0, 1, (2000, 5)).astype(np.float32)
+    A = rng.normal(0, 1, (5, 3))
+    t = (np.maximum(x @ A, 0) - 0.5 * x[:, :1]).astype(np.float32)
+    p0 = G.mlp_init(1)
+    l0, _ = G.mlp_loss_grad(p0, x, t)
+    p, l1 = G.mlp_train(p0, x, t, epochs=800, lr=3e-3)
+    assert l1 < 0.05 * l0
+    # deterministic: the same run gives the same bits
+    p2, l2 = G.mlp_train(p0, x, t, epochs=800, lr=3e-3)
+    assert np.array_equal(p, p2) and l1 == l2
+
+
+def test_policy_slots_and_errors():
+    ctx = G.default_context()
+    for slot in (G.POLICY_SINGLE, G.POLICY_BATCHED):
+        G.policy_clear(ctx, slot)
+        assert G.policy_get(ctx, slot) is None
+        p = _params(10 + slot)
+        G.policy_set(p, ctx, slot)
+        assert np.array_equal(G.policy_get(ctx, slot), p)
+    with pytest.raises(ValueError):
+        G.policy_set(np.zeros(10, np.float32), ctx)
+    bad = _params(1)
+    bad[7] = np.nan
+    with pytest.raises(ValueError):
+        G.policy_set(bad, ctx)
+    with pytest.raises(ValueError):
+        G.policy_set(_params(1), ctx, slot=5)
+    for slot in (G.POLICY_SINGLE, G.POLICY_BATCHED):
+        G.policy_clear(ctx, slot)
+
+
+def test_recorders_time_every_level():
+    g = G.Graph.named("rmat12").symmetrize()
+    bfs = G.GpuBfs(g)
+    md = g.m / g.n
+    ref = bfs.run("gpu_do", 5, G.bfs_options())
+    feat, times, tr = bfs.record_levels(5)
+    assert len(tr) == len(ref.trace)
+    assert np.all(times > 0)
+    for L, f in zip(tr, feat):
+        np.testing.assert_array_equal(f, G.policy_features(L.frontier, L.frontier_edges, L.unvisited_edges,
+                                                           L.level, md))
+    src = np.array(g.pick_sources(64, 3), dtype=np.uint32)
+    _, res, _ = bfs.run_batch(src, G.bfs_options(), fetch=False)
+    feat, times, tr = bfs.record_batch_levels(src)
+    assert len(tr) == res.levels and len(feat) == res.levels
+    assert np.all(times > 0)
+    assert tr[0].frontier == len(set(src.tolist()))
+    assert feat[0][3] == 0.0  # level feature of level 0
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_learned_weights_keep_depths_exact_and_follow_the_forward_kernel(seed):
+    ctx = G.default_context()
+    g = G.Graph.named("grid100x100" if seed % 2 else "rmat12").symmetrize()
+    bfs = G.GpuBfs(g, ctx)
+    p = _params(100 + seed) * (1 + seed)
+    G.policy_set(p, ctx, G.POLICY_BATCHED)
+    G.policy_set(p, ctx, G.POLICY_SINGLE)
+    try:
+        src = np.array(g.pick_sources(70, seed), dtype=np.uint32)
+        dist, res, tr = bfs.run_batch(src, G.bfs_options(policy=G.POLICY_LEARNED), trace=True)
+        for i in range(0, len(src), 9):
+            assert np.array_equal(dist[i], O.bfs_serial(g.row_ptr, g.col_idx, int(src[i])))
+        # the boundary's inference picks the argmin of the forward kernel on the same features
+        feat, _, _ = bfs.record_batch_levels(src)
+        y = G.mlp_forward(p, feat)
+        assert [L.action for L in tr] == np.argmin(y, axis=1).tolist()
+        r = bfs.run("gpu_do", int(src[0]), G.bfs_options(policy=G.POLICY_LEARNED))
+        assert np.array_equal(r.dist, O.bfs_serial(g.row_ptr, g.col_idx, int(src[0])))
+    finally:
+        G.policy_clear(ctx, G.POLICY_BATCHED)
+        G.policy_clear(ctx, G.POLICY_SINGLE)
+
+
+def test_trained_batched_policy_gains_on_held_out_graphs():
+    """The committed batched weights (trained by policy/train_policy.py on 50
+    seeded graphs) against the GAP heuristic on graphs outside the training
+    seeds: never slower overall, depths still exact."""
+    p = G.load_policy_file(G.POLICY_BATCHED)
+    assert p is not None, "graph_analytics_applesilicon_amd/policy/batched.f32 missing"
+    ctx = G.default_context()
+    G.policy_set(p, ctx, G.POLICY_BATCHED)
+    try:
+        graphs = [G.Graph.rmat(18, 16, 0.49, 0.23, 0.23, seed=7001).permute(G.random_perm(1 << 18, 7001)),
+                  G.Graph.grid(256, 512, 0.7, 7002).largest_component()[0]]
+        total = {G.POLICY_HEURISTIC: 0.0, G.POLICY_LEARNED: 0.0}
+        for g in graphs:
+            bfs = G.GpuBfs(g, ctx)
+            src = np.array(g.pick_sources(64, 7), dtype=np.uint32)
+            for pol in total:
+                o = G.bfs_options(policy=pol)
+                dist, _, _ = bfs.run_batch(src, o)
+                for i in (0, 31, 63):
+                    assert np.array_equal(dist[i], O.bfs_serial(g.row_ptr, g.col_idx, int(src[i])))
+                total[pol] += float(np.median([bfs.run_batch(src, o, fetch=False)[1].gpu_ms for _ in range(5)]))
+        assert total[G.POLICY_LEARNED] <= 1.05 * total[G.POLICY_HEURISTIC], total
+    finally:
+        G.policy_clear(ctx, G.POLICY_BATCHED)
