@@ -13,7 +13,12 @@ the plain per-source gpu_do rate, PageRank time to L1 < 1e-4 against the FP64
 oracle, the roofline of the dominant kernel (HIP events on the library's
 stream) and the reference's own cpu_do / cpu_omp timed on this host.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-pagerank]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-pagerank] [--only bfs|pagerank]
+       python bench.py --copy-calibration   (a plain device copy of 1 GiB, three times: the
+                                             reference kernel for the FETCH_SIZE / WRITE_SIZE scale)
+The roofline's `traffic` (HBM bytes per launch) is read from
+profiles/hbm_traffic.json, which holds the rocprofv3 FETCH_SIZE and WRITE_SIZE
+readings of the dominant kernels scaled by the copy calibration.
 """
 import argparse
 import json
@@ -27,6 +32,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+COPY_BYTES = 1 << 30
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed counter runs, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rows = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    for r in rows:
+        if r.get("kernel") == kernel and r.get("workload") == workload:
+            return r.get("traffic_bytes")
+    return None
+
+
+def copy_calibration():
+    """One plain device-to-device copy of COPY_BYTES, three times (torch's copy kernel)."""
+    import torch
+    x = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": COPY_BYTES, "launches": 3}), flush=True)
 
 
 def log(msg):
@@ -45,7 +76,14 @@ def main():
                     help="graph of the PageRank part (default: the scale-23 heavy-skew RMAT)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="auto", choices=["auto", "batched", "sequential"])
+    ap.add_argument("--only", default="all", choices=["all", "bfs", "pagerank"],
+                    help="profiling runs: just one part (the JSON line then carries that part)")
+    ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-ref", action="store_true", help="skip the FP64 PageRank reference (profiling runs)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build runs on a single "
               "MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
@@ -54,6 +92,61 @@ def main():
     import graph_analytics_applesilicon_amd as G
     from oracle import pyoracle as O
 
+    ctx = G.default_context()
+    name, cus, _ = ctx.info()
+    bfs_res = pr_res = cpu = None
+    if args.only in ("all", "bfs"):
+        bfs_res = bfs_part(G, O, args, ctx)
+    if args.only == "pagerank" or (args.only == "all" and not args.no_pagerank):
+        pg = G.Graph.named(args.pr_graph)
+        pr_res = pagerank_part(G, O, pg, ctx, args.pr_graph, with_ref=not args.no_ref)
+        pr_res["graph"] = args.pr_graph
+    if bfs_res is not None and not args.no_cpu:
+        cpu = cpu_baseline(O, bfs_res.pop("_graph"), bfs_res.pop("_sources"), bfs_res["comp_edges"])
+    if bfs_res is not None:
+        bfs_res.pop("_graph", None)
+        bfs_res.pop("_sources", None)
+
+    if bfs_res is not None:
+        line = {
+            "metric": "BFS GTEPS batched+learned vs plain gpu_do; PageRank ms to L1<1e-4 vs FP64",
+            "value": bfs_res["value"],
+            "unit": "GTEPS",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": bfs_res["ms_per_step"],
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "u32",
+            "data": "synthetic",
+            "config": {"workload": f"{args.graph}: undirected RMAT scale 22, ef 8, a/b/c .57/.19/.19, ids "
+                                   f"permuted (seed 2); {bfs_res['sources']} seeded LCC sources per step",
+                       "n": bfs_res["n"], "m_arcs": bfs_res["m"], "sources": bfs_res["sources"],
+                       "path": bfs_res["path"], "device": name, "cus": cus},
+            "roofline": bfs_res.pop("roofline"),
+            "cpu_baseline": cpu,
+        }
+        for k in ("value", "ms_per_step", "n", "m", "sources", "path"):
+            bfs_res.pop(k, None)
+        line["bfs"] = bfs_res
+        if pr_res is not None:
+            line["pagerank_roofline"] = pr_res.pop("roofline")
+            line["pagerank"] = pr_res
+    else:
+        jac = pr_res["jacobi_certified"]
+        line = {
+            "metric": "PageRank ms to L1<1e-4 vs FP64 (certified Jacobi)",
+            "value": jac["ms"], "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": jac["ms"],
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"{args.pr_graph}: RMAT scale 23, ef 16, a/b/c .60/.17/.17", "device": name},
+            "roofline": pr_res.pop("roofline"), "cpu_baseline": None, "pagerank": pr_res,
+        }
+    print(json.dumps(line), flush=True)
+
+
+def bfs_part(G, O, args, ctx):
     t0 = time.time()
     g = G.Graph.named(args.graph)
     n, m = g.n, g.m
@@ -63,12 +156,9 @@ def main():
     log(f"graph {args.graph}: n {n} m {m} (lcc undirected edges {comp_edges}), {len(sources)} sources, "
         f"generated in {time.time() - t0:.1f}s")
     del lcc
-
-    ctx = G.default_context()
-    name, cus, hbm = ctx.info()
     bfs = G.GpuBfs(g, ctx)
     rp, ci = g.row_ptr, g.col_idx
-    out = {}
+    out = {"comp_edges": comp_edges, "n": n, "m": m, "sources": len(sources), "_graph": g, "_sources": sources}
 
     # ---------------------------------------------------------- plain gpu_do
     plain_opts = G.bfs_options(time_levels=0)
@@ -80,114 +170,80 @@ def main():
     plain_s = time.perf_counter() - t
     plain_gteps = len(sources) * comp_edges / plain_s / 1e9
     log(f"plain gpu_do: {len(sources)} sequential traversals {plain_s * 1e3:.2f} ms, {plain_gteps:.2f} GTEPS")
-    # parity of the plain path on a few sources
     bad = 0
     for s in sources[:4]:
         d = bfs.run("gpu_do", int(s), plain_opts, trace=False).dist
         bad += int(not np.array_equal(d, O.bfs_serial(rp, ci, int(s))))
-    out["plain_gpu_do"] = {"gteps": plain_gteps, "ms_per_64": plain_s * 1e3 * 64 / len(sources),
+    out["plain_gpu_do"] = {"gteps": plain_gteps, "ms_per_batch": plain_s * 1e3,
                            "bit_exact_sources_checked": 4, "mismatches": bad}
 
     # ---------------------------------------------------------- timed steps
-    batched = args.mode in ("auto", "batched") and hasattr(ctx.lib, "ggpu_msbfs_run")
-    bopts = G.bfs_options(policy=G.POLICY_LEARNED) if batched else plain_opts
+    batched = args.mode in ("auto", "batched")
+    bopts = G.bfs_options(policy=G.POLICY_LEARNED)
     if batched:
-        pol = G.load_policy_file()
-        if pol is not None:
-            G.policy_set(pol, ctx)
+        installed = G.install_policies(ctx)
+        out["learned_policy_installed"] = bool(installed[G.POLICY_BATCHED])
 
     def step():
         if batched:
             _, res, _ = bfs.run_batch(sources, bopts, fetch=False)
             return res.gpu_ms
-        gms = 0.0
         for s in sources:
             bfs.run("gpu_do", int(s), plain_opts, trace=False, fetch=False)
-        return gms
+        return 0.0
 
     for _ in range(args.warmup):
         step()
     ctx.sync()
     t = time.perf_counter()
-    kernel_ms = []
-    for _ in range(args.steps):
-        kernel_ms.append(step())
+    kernel_ms = [step() for _ in range(args.steps)]
     ctx.sync()
     total_s = time.perf_counter() - t
     ms_per_step = total_s * 1e3 / args.steps
     value = len(sources) * comp_edges / (ms_per_step / 1e3) / 1e9
-    log(f"timed: {args.steps} steps, {ms_per_step:.3f} ms/step, {value:.2f} GTEPS ({'batched' if batched else 'sequential gpu_do'})")
+    log(f"timed: {args.steps} steps, {ms_per_step:.3f} ms/step, {value:.2f} GTEPS "
+        f"({'batched' if batched else 'sequential gpu_do'})")
+    out.update({"value": value, "ms_per_step": ms_per_step,
+                "path": "batched BFS, one persistent launch, learned policy" if batched else "64 sequential gpu_do",
+                "speedup_vs_plain_gpu_do": value / plain_gteps if plain_gteps else None})
 
-    # parity of the measured path
-    if batched:
-        dist, res, _ = bfs.run_batch(sources, bopts, fetch=True)
-        mism = 0
-        for i in range(0, len(sources), max(1, len(sources) // 8)):
-            mism += int(not np.array_equal(dist[i], O.bfs_serial(rp, ci, int(sources[i]))))
-        out["parity"] = {"sources_checked": len(range(0, len(sources), max(1, len(sources) // 8))),
-                         "mismatches": mism}
-
-    # ---------------------------------------------------------- roofline
     roof = None
     if batched:
+        dist, res, _ = bfs.run_batch(sources, bopts, fetch=True)
+        checked = list(range(0, len(sources), max(1, len(sources) // 8)))
+        mism = sum(int(not np.array_equal(dist[i], O.bfs_serial(rp, ci, int(sources[i])))) for i in checked)
+        out["parity"] = {"sources_checked": len(checked), "mismatches": mism, "levels": res.levels,
+                         "level_actions": list(res.level_actions)}
+        heur = G.bfs_options(policy=G.POLICY_HEURISTIC)
+        bfs.run_batch(sources, heur, fetch=False)
+        hms = float(np.median([bfs.run_batch(sources, heur, fetch=False)[1].gpu_ms for _ in range(3)]))
+        out["heuristic_batched_ms"] = hms
+        # S = 256 sources in one launch (the second batch size of the workload)
+        s256 = g.pick_sources(256, 2)
+        bfs.run_batch(s256, bopts, fetch=False)
+        ms256 = float(np.median([bfs.run_batch(s256, bopts, fetch=False)[1].gpu_ms for _ in range(3)]))
+        out["batch256"] = {"sources": len(s256), "gpu_ms": ms256,
+                           "gteps": len(s256) * comp_edges / (ms256 / 1e3) / 1e9}
         # one persistent launch = one batch: 4(n+1) + 4m + 4*n*S algorithmic bytes
         bytes_batch = 4 * (n + 1) + 4 * m + 4 * n * len(sources)
         avg_ms = float(np.mean(kernel_ms))
         ach = bytes_batch / (avg_ms / 1e3) / 1e9
-        roof = {"kernel": "k_msbfs (persistent batch)", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+        roof = {"kernel": "k_msbfs<1> (persistent batched BFS)", "bound": "hbm", "achieved": ach,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": measured_traffic("k_msbfs<1>", f"{args.graph}/k{len(sources)}"),
                 "bytes_per_launch": bytes_batch, "avg_launch_ms": avg_ms}
-
-    # ---------------------------------------------------------- PageRank
-    if not args.no_pagerank:
-        del bfs
-        pg = g if args.pr_graph == args.graph else G.Graph.named(args.pr_graph)
-        out["pagerank"] = pagerank_part(G, O, pg, ctx)
-        out["pagerank"]["graph"] = args.pr_graph
-        if roof is None:
-            roof = out["pagerank"].pop("roofline")
-        else:
-            out["pagerank_roofline"] = out["pagerank"].pop("roofline")
-
-    # ---------------------------------------------------------- CPU baseline
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(O, g, sources, comp_edges)
-
-    line = {
-        "metric": "BFS GTEPS batched+learned vs plain gpu_do; PageRank ms to L1<1e-4 vs FP64",
-        "value": value,
-        "unit": "GTEPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic",
-        "config": {"workload": f"{args.graph}: undirected RMAT scale 22, ef 8, a/b/c .57/.19/.19, ids permuted "
-                               f"(seed 2); {len(sources)} seeded LCC sources per step",
-                   "n": n, "m_arcs": m, "sources": len(sources),
-                   "path": "batched MS-BFS (one persistent launch)" if batched else "64 sequential gpu_do",
-                   "device": name, "cus": cus},
-        "speedup_vs_plain_gpu_do": value / plain_gteps if plain_gteps else None,
-        "roofline": roof,
-        "cpu_baseline": cpu,
-    }
-    line.update(out)
-    print(json.dumps(line), flush=True)
+    out["roofline"] = roof
+    return out
 
 
-def pagerank_part(G, O, g, ctx):
+def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
     """Time to L1 < 1e-4 against the FP64 oracle, and the per-sweep roofline."""
     n, m = g.n, g.m
     od = g.degrees()
     t = g if not g.directed else g.transpose()
     pr = G.GpuPageRank(t, od, ctx)  # undirected: the graph is its own transpose
     ref = None
-    rlib = O.ref_lib()
+    rlib = O.ref_lib() if with_ref else None
     t0 = time.time()
     if rlib is not None:
         ref, it, _ = O.ref_pagerank(t.row_ptr, t.col_idx, od, 0.85, 1000, 1e-10, parallel=True)
@@ -223,7 +279,8 @@ def pagerank_part(G, O, g, ctx):
     ach = bytes_sweep / (per_sweep_ms / 1e3) / 1e9
     res["roofline"] = {"kernel": f"k_pr_sweep (gpu_pull_adaptive, Jacobi, stream layout, {best_name} order)",
                        "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": ach / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bytes_sweep,
+                       "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("k_pr_sweep", g_name),
+                       "bytes_per_launch": bytes_sweep,
                        "avg_launch_ms": per_sweep_ms}
     log(f"pagerank: certified Jacobi {res['jacobi_certified']['ms']:.2f} ms "
         f"({res['jacobi_certified']['iters']} it), GS {res['gs_tol_1e-6']['ms']:.2f} ms "
