@@ -76,10 +76,11 @@ def main():
                     help="graph of the PageRank part (default: the scale-23 heavy-skew RMAT)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="auto", choices=["auto", "batched", "sequential"])
-    ap.add_argument("--only", default="all", choices=["all", "bfs", "pagerank"],
+    ap.add_argument("--only", default="all", choices=["all", "bfs", "pagerank", "loops"],
                     help="profiling runs: just one part (the JSON line then carries that part)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 PageRank reference (profiling runs)")
+    ap.add_argument("--no-loops", action="store_true", help="skip the level-loop comparison on the grid graph")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -94,9 +95,11 @@ def main():
 
     ctx = G.default_context()
     name, cus, _ = ctx.info()
-    bfs_res = pr_res = cpu = None
+    bfs_res = pr_res = cpu = loops = None
     if args.only in ("all", "bfs"):
         bfs_res = bfs_part(G, O, args, ctx)
+    if args.only == "loops" or (args.only == "all" and not args.no_loops):
+        loops = loops_part(G, O, ctx)
     if args.only == "pagerank" or (args.only == "all" and not args.no_pagerank):
         pg = G.Graph.named(args.pr_graph)
         pr_res = pagerank_part(G, O, pg, ctx, args.pr_graph, with_ref=not args.no_ref)
@@ -131,9 +134,18 @@ def main():
         for k in ("value", "ms_per_step", "n", "m", "sources", "path"):
             bfs_res.pop(k, None)
         line["bfs"] = bfs_res
+        if loops is not None:
+            line["level_loops_grid_c3"] = loops
         if pr_res is not None:
             line["pagerank_roofline"] = pr_res.pop("roofline")
             line["pagerank"] = pr_res
+    elif pr_res is None:
+        line = {"metric": "BFS us per level on the grid graph (gpu_do)", "value": loops["persistent"]["us_per_level"],
+                "unit": "us/level", "n_gpus": 1, "steps": 1, "warmup": 1,
+                "ms_per_step": loops["persistent"]["ms_per_traversal"], "higher_is_better": False,
+                "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+                "config": {"workload": loops["workload"], "device": name}, "roofline": None,
+                "cpu_baseline": None, "level_loops_grid_c3": loops}
     else:
         jac = pr_res["jacobi_certified"]
         line = {
@@ -233,6 +245,37 @@ def bfs_part(G, O, args, ctx):
                 "traffic": measured_traffic("k_msbfs<1>", f"{args.graph}/k{len(sources)}"),
                 "bytes_per_launch": bytes_batch, "avg_launch_ms": avg_ms}
     out["roofline"] = roof
+    return out
+
+
+def loops_part(G, O, ctx, trials=3):
+    """gpu_do on the grid graph (>1500 levels) under each level loop: ms per traversal and us per level."""
+    g = G.Graph.named("grid_c3")
+    bfs = G.GpuBfs(g, ctx)
+    src = [int(s) for s in g.pick_sources(4, 3)]
+    out = {"workload": f"grid_c3: largest component of a 1024 x 2048 grid, edges kept with p = 0.7 (seed 3); "
+                       f"n {g.n}, m {g.m}; gpu_do from {len(src)} pickSources sources (seed 3)"}
+    modes = [("host", dict(loop=G.LOOP_HOST, time_levels=0), "gpu_do"),
+             ("graph_k8", dict(loop=G.LOOP_GRAPH, graph_levels=8), "gpu_do"),
+             ("graph_k64", dict(loop=G.LOOP_GRAPH, graph_levels=64), "gpu_do"),
+             ("persistent", dict(loop=G.LOOP_PERSISTENT), "gpu_do"),
+             ("hybrid", dict(loop=G.LOOP_HOST, time_levels=0), "hybrid")]
+    ref = O.bfs_serial(g.row_ptr, g.col_idx, src[0])
+    for key, kw, variant in modes:
+        o = G.bfs_options(**kw)
+        r = bfs.run(variant, src[0], o, trace=False)
+        exact = bool(np.array_equal(r.dist, ref))
+        times, levels = [], 0
+        for s in src:
+            for _ in range(trials):
+                t = time.perf_counter()
+                r = bfs.run(variant, s, o, trace=False, fetch=False)
+                times.append((time.perf_counter() - t) * 1e3)
+                levels = r.levels
+        ms = float(np.median(times))
+        out[key] = {"ms_per_traversal": ms, "levels": levels, "us_per_level": ms * 1e3 / max(levels, 1),
+                    "bit_exact": exact}
+        log(f"  grid_c3 {key}: {ms:.2f} ms per traversal, {levels} levels, {ms * 1e3 / max(levels, 1):.1f} us/level")
     return out
 
 

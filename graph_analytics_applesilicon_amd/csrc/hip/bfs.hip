@@ -151,6 +151,7 @@ BfsEngine::BfsEngine(ggpu_ctx* c, uint64_t n_, uint64_t m_, const uint32_t* hrp,
 
 BfsEngine::~BfsEngine() {
     if (ms_state && ms_free) ms_free(ms_state);
+    if (dl_state && dl_free_fn) dl_free_fn(dl_state);
     for (auto& e : events) (void)hipEventDestroy(e);
     if (h_mirror) (void)hipHostFree(h_mirror);
 }

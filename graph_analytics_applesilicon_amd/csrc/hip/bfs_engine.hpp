@@ -86,8 +86,36 @@ __host__ __device__ inline void mlp_forward1(const float* p, const float* x, flo
     }
 }
 
+// The same forward pass on the 64 lanes of one wave: lane j computes hidden
+// unit j of each layer and lane k output k, every sum in mlp_forward1's order
+// (so the results are bit-identical); the other layer's values arrive by
+// shuffles. All 64 lanes must call it; every lane returns y[0..2].
+__device__ inline void mlp_forward_wave(const float* p, const float* x, float* y) {
+    const float* W1 = p;
+    const float* b1 = W1 + 32 * 5;
+    const float* W2 = b1 + 32;
+    const float* b2 = W2 + 32 * 32;
+    const float* W3 = b2 + 32;
+    const float* b3 = W3 + 3 * 32;
+    const int lane = (int)__lane_id();
+    const int j = lane & 31;
+    float a = b1[j];
+    for (int i = 0; i < 5; ++i) a += W1[j * 5 + i] * x[i];
+    const float h1 = a > 0.f ? a : 0.f;
+    a = b2[j];
+    for (int i = 0; i < 32; ++i) a += W2[j * 32 + i] * __shfl(h1, i, 64);
+    const float h2 = a > 0.f ? a : 0.f;
+    const int k = lane < 3 ? lane : 0;
+    a = b3[k];
+    for (int i = 0; i < 32; ++i) a += W3[k * 32 + i] * __shfl(h2, i, 64);
+    for (int q = 0; q < 3; ++q) y[q] = __shfl(a, q, 64);
+}
+
 // Picks the action of the level about to run (state after the previous level).
-__host__ __device__ inline void decide_action(LoopState& st, const PolicyView& pv, uint64_t n) {
+// y_pre: the MLP's outputs for this state when the caller already evaluated
+// them (the device loops do, on one wave); otherwise computed here.
+__host__ __device__ inline void decide_action(LoopState& st, const PolicyView& pv, uint64_t n,
+                                              const float* y_pre = nullptr) {
     uint32_t act;
     if (pv.variant == GGPU_BFS_EDGE) {
         act = GGPU_ACTION_TD_EDGE;
@@ -99,8 +127,12 @@ __host__ __device__ inline void decide_action(LoopState& st, const PolicyView& p
         st.bottom_up = act == GGPU_ACTION_BU;
     } else if (pv.mode == GGPU_POLICY_LEARNED && pv.mlp) {
         float f[5], y[3];
-        policy_features(st.size, st.total, st.unvisited_edges, st.level, pv.mean_degree, f);
-        mlp_forward1(pv.mlp, f, y);
+        if (y_pre) {
+            for (int q = 0; q < 3; ++q) y[q] = y_pre[q];
+        } else {
+            policy_features(st.size, st.total, st.unvisited_edges, st.level, pv.mean_degree, f);
+            mlp_forward1(pv.mlp, f, y);
+        }
         act = 0;
         for (uint32_t k = 1; k < 3; ++k)
             if (y[k] < y[act]) act = k;
@@ -172,11 +204,9 @@ struct BfsEngine {
     HybridMirror hmirror;
     bool mirror_ready = false;
     void* h_mirror = nullptr;
-    // device-driven loop resources (bfs_loop.hip)
-    DBuf<LoopState> dstate;
-    DBuf<uint8_t> dactions;
-    DBuf<ggpu_level> dtrace;
-    DBuf<unsigned int> barrier;
+    // device-driven loop resources (bfs_loop.hip), released through dl_free_fn
+    void* dl_state = nullptr;
+    void (*dl_free_fn)(void*) = nullptr;
     // batched BFS state (msbfs.hip), released through ms_free
     void* ms_state = nullptr;
     void (*ms_free)(void*) = nullptr;
@@ -223,7 +253,7 @@ struct BfsEngine {
     }
 };
 
-inline ggpu_level trace_record(const LoopState& st, const LevelOut& r, char device, double ms) {
+__host__ __device__ inline ggpu_level trace_record(const LoopState& st, const LevelOut& r, char device, double ms) {
     ggpu_level L{};
     L.level = st.level;
     L.action = st.action;

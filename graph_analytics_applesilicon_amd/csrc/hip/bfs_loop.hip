@@ -1,11 +1,525 @@
-// Device-driven BFS level loops (filled in below).
+// Device-driven single-source BFS level loops (no reference counterpart; the
+// reference syncs the host every level, src/metal/bfs_gpu.mm:83-104):
+//   GGPU_LOOP_PERSISTENT  one cooperative launch runs every level; levels are
+//                         separated by the grid barrier of grid_sync.hpp;
+//   GGPU_LOOP_GRAPH       a captured hipGraph of K levels (a decide kernel and a
+//                         level kernel per level) is replayed until the
+//                         frontier is empty; the host checks once per K levels.
+// Both make exactly the host loop's decisions: every block (persistent) or the
+// decide kernel (graph) runs decide_action / fold_level of bfs_engine.hpp on
+// the same per-level counters, so the trace (frontier, direction) equals the
+// host loop's and the oracle's.
+//
+// State: dist[n] (INF before the launch), visited bitmap vis, frontier
+// bitmaps in three slots (level l reads slot l%3, writes (l+1)%3 and clears
+// (l+2)%3, the slot of level l-1). One phase per level:
+//   top-down: each frontier vertex pushes to its neighbours; atomicOr on vis
+//     returns whether this push discovered the vertex (exactly once);
+//   bottom-up: each unvisited vertex scans its neighbours for a frontier bit.
+// Vertices of degree >= kHeavyMin are handled as kHeavyChunk-edge wave items
+// in both directions (the engine's heavy list), the rest lane per vertex.
+// Writes read by other blocks after a barrier are atomics or write-through
+// stores; the persistent loop's barrier invalidates only L1, so data other
+// blocks wrote (counters, frontier and visited words) is read with coherent
+// loads while the read-only CSR stays cached (grid_sync.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
 #include "bfs_engine.hpp"
+#include "grid_sync.hpp"
+
+namespace ggpu {
+
+constexpr uint32_t kDlThreads = 1024;      // persistent: one block per CU
+constexpr uint32_t kDlLevelThreads = 256;  // graph mode level kernel
+constexpr uint32_t kDlParts = 4;           // counters per block: found, next_edges, examined, pad
+
+// Per-run parameters and the live loop state, in device memory (the graph's
+// kernels read them; their arguments are fixed at capture).
+struct DlState {
+    LoopState st;
+    PolicyView pv;
+    uint32_t started, prev_action, levels, done;
+    uint64_t t_prev;
+    double tick_ms;
+};
+
+struct DlDev {
+    const uint32_t* rp;
+    const uint32_t* ci;
+    uint32_t* dist;
+    uint64_t* vis;                  // nb
+    uint64_t* fb;                   // 3 x nb
+    unsigned long long* part;       // 2 x grid x kDlParts
+    const uint4* heavy;             // {v, first edge, end edge, degree}, heaviest first
+    DlState* ds;
+    ggpu_level* trace;              // null: no trace
+    uint32_t* bar;                  // persistent barrier words, then abort
+    uint32_t* abort;
+    uint32_t n_heavy, n, nb, trace_cap;
+    uint64_t m;
+};
+
+struct DlAcc {
+    unsigned long long found = 0, next_edges = 0, examined = 0;
+};
+
+__device__ __forceinline__ uint32_t dl_gwave() { return (blockIdx.x * blockDim.x + threadIdx.x) / kWave; }
+__device__ __forceinline__ uint32_t dl_nwaves() { return gridDim.x * blockDim.x / kWave; }
+
+// u reached from the frontier at depth `depth`: the atomicOr on vis decides
+// which pusher discovered it.
+__device__ __forceinline__ void dl_discover(const DlDev& D, uint64_t* fbn, uint32_t u, uint32_t depth, DlAcc& a) {
+    const uint64_t bit = 1ull << (u & 63);
+    if (D.vis[u >> 6] & bit) return;  // visited before this level (a stale 0 only costs an atomic)
+    const uint64_t old = atomicOr((unsigned long long*)&D.vis[u >> 6], (unsigned long long)bit);
+    if (old & bit) return;
+    store_through(&D.dist[u], depth);
+    atomicOr((unsigned long long*)&fbn[u >> 6], (unsigned long long)bit);
+    a.found += 1;
+    a.next_edges += D.rp[u + 1] - D.rp[u];
+}
+
+__device__ void dl_top_down(const DlDev& D, uint32_t level, bool lane_per_vertex, DlAcc& a) {
+    const uint64_t* fb = D.fb + (size_t)(level % 3) * D.nb;
+    uint64_t* fbn = D.fb + (size_t)((level + 1) % 3) * D.nb;
+    const uint32_t lane = lane_id(), depth = level + 1;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // light frontier vertices: each lane owns a frontier word, one vertex per round
+    for (uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x; wi - lane < D.nb; wi += stride) {
+        uint64_t fw = wi < D.nb ? load_coherent(&fb[wi]) : 0ull;
+        while (__ballot(fw != 0)) {
+            uint32_t v = 0xFFFFFFFFu, b = 0, e = 0;
+            if (fw) {
+                v = wi * 64 + (uint32_t)(__ffsll((long long)fw) - 1);
+                fw &= fw - 1;
+                b = D.rp[v];
+                e = D.rp[v + 1];
+                if (e - b >= kHeavyMin) v = 0xFFFFFFFFu;  // heavy: pushed by its items below
+            }
+            if (lane_per_vertex) {
+                if (v != 0xFFFFFFFFu)
+                    for (uint32_t x = b; x < e; ++x) dl_discover(D, fbn, D.ci[x], depth, a);
+            } else {
+                uint64_t todo = __ballot(v != 0xFFFFFFFFu);
+                while (todo) {
+                    const int j = __ffsll((long long)todo) - 1;
+                    todo &= todo - 1;
+                    const uint32_t bj = __shfl(b, j, kWave), ej = __shfl(e, j, kWave);
+                    for (uint32_t x = bj + lane; x < ej; x += kWave) dl_discover(D, fbn, D.ci[x], depth, a);
+                }
+            }
+        }
+    }
+    // heavy frontier vertices: 64 items nwaves apart per wave, the frontier ones pushed in turn
+    for (uint64_t r = 0; dl_gwave() + r * 64 * dl_nwaves() < D.n_heavy; ++r) {
+        const uint64_t it = dl_gwave() + (r * 64 + lane) * dl_nwaves();
+        uint4 item = make_uint4(0, 0, 0, 0);
+        bool want = false;
+        if (it < D.n_heavy) {
+            item = D.heavy[it];
+            want = (load_coherent(&fb[item.x >> 6]) >> (item.x & 63)) & 1ull;
+        }
+        uint64_t todo = __ballot(want);
+        while (todo) {
+            const int j = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            const uint32_t y = __shfl(item.y, j, kWave), z = __shfl(item.z, j, kWave);
+            for (uint32_t x = y + lane; x < z; x += kWave) dl_discover(D, fbn, D.ci[x], depth, a);
+        }
+    }
+}
+
+__device__ void dl_bottom_up(const DlDev& D, uint32_t level, DlAcc& a) {
+    const uint64_t* fb = D.fb + (size_t)(level % 3) * D.nb;
+    uint64_t* fbn = D.fb + (size_t)((level + 1) % 3) * D.nb;
+    const uint32_t lane = lane_id(), depth = level + 1;
+    // light vertices: lane per vertex, first frontier neighbour ends the scan
+    for (uint32_t wd = dl_gwave(); wd < D.nb; wd += dl_nwaves()) {
+        const uint32_t v = wd * 64 + lane;
+        const uint64_t seen = load_coherent(&D.vis[wd]);
+        bool found = false;
+        if (v < D.n && !((seen >> lane) & 1ull)) {
+            const uint32_t b = D.rp[v], e = D.rp[v + 1];
+            if (e > b && e - b < kHeavyMin) {
+                for (uint32_t x = b; x < e && !found; x += 4) {
+                    const uint32_t c = min(4u, e - x);
+                    uint32_t u[4];
+#pragma unroll
+                    for (uint32_t i = 0; i < 4; ++i) u[i] = i < c ? D.ci[x + i] : 0u;
+                    uint32_t hit = 4;
+#pragma unroll
+                    for (int i = 3; i >= 0; --i)
+                        if ((uint32_t)i < c && ((load_coherent(&fb[u[i] >> 6]) >> (u[i] & 63)) & 1ull)) hit = (uint32_t)i;
+                    if (hit < 4) {
+                        found = true;
+                        a.examined += hit + 1;
+                    } else {
+                        a.examined += c;
+                    }
+                }
+                if (found) {
+                    store_through(&D.dist[v], depth);
+                    a.next_edges += e - b;
+                }
+            }
+        }
+        const uint64_t mask = __ballot(found);
+        if (lane == 0 && mask) {
+            atomicOr((unsigned long long*)&D.vis[wd], (unsigned long long)mask);
+            atomicOr((unsigned long long*)&fbn[wd], (unsigned long long)mask);
+            a.found += (uint64_t)__popcll(mask);
+        }
+    }
+    // heavy vertices: items of unvisited vertices, scanned 256 edges at a time until a frontier hit
+    for (uint64_t r = 0; dl_gwave() + r * 64 * dl_nwaves() < D.n_heavy; ++r) {
+        const uint64_t it = dl_gwave() + (r * 64 + lane) * dl_nwaves();
+        uint4 item = make_uint4(0, 0, 0, 0);
+        bool want = false;
+        if (it < D.n_heavy) {
+            item = D.heavy[it];
+            want = !((load_coherent(&D.vis[item.x >> 6]) >> (item.x & 63)) & 1ull);
+        }
+        uint64_t todo = __ballot(want);
+        while (todo) {
+            const int j = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            const uint32_t v = __shfl(item.x, j, kWave), y = __shfl(item.y, j, kWave), z = __shfl(item.z, j, kWave);
+            bool any = false;
+            for (uint32_t xb = y; xb < z && !any; xb += 4 * kWave) {
+                bool hit = false;
+#pragma unroll
+                for (uint32_t i = 0; i < 4; ++i) {
+                    const uint32_t x = xb + i * kWave + lane;
+                    if (x < z) {
+                        const uint32_t u = D.ci[x];
+                        hit |= ((load_coherent(&fb[u >> 6]) >> (u & 63)) & 1ull) != 0;
+                        a.examined += 1;
+                    }
+                }
+                any = __ballot(hit) != 0;
+            }
+            if (any && lane == 0) {
+                const uint64_t bit = 1ull << (v & 63);
+                const uint64_t old = atomicOr((unsigned long long*)&D.vis[v >> 6], (unsigned long long)bit);
+                if (!(old & bit)) {
+                    store_through(&D.dist[v], depth);
+                    atomicOr((unsigned long long*)&fbn[v >> 6], (unsigned long long)bit);
+                    a.found += 1;
+                    a.next_edges += D.rp[v + 1] - D.rp[v];
+                }
+            }
+        }
+    }
+}
+
+// One level: the action's phase, the clear of the slot of level - 1, the block's counters.
+__device__ void dl_level(const DlDev& D, uint32_t level, uint32_t action, unsigned long long* part_slot) {
+    DlAcc a;
+    if (action == GGPU_ACTION_BU) dl_bottom_up(D, level, a);
+    else dl_top_down(D, level, action == GGPU_ACTION_TD_BINNED, a);
+    uint64_t* old = D.fb + (size_t)((level + 2) % 3) * D.nb;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < D.nb; i += gridDim.x * blockDim.x)
+        store_through(&old[i], 0ull);
+    __shared__ unsigned long long s_red[kDlThreads / kWave];
+    const unsigned long long f = block_sum(a.found, s_red);
+    const unsigned long long ne = block_sum(a.next_edges, s_red);
+    const unsigned long long ex = block_sum(a.examined, s_red);
+    if (threadIdx.x == 0) {
+        unsigned long long* p = part_slot + (size_t)blockIdx.x * kDlParts;
+        store_through(&p[0], f);
+        store_through(&p[1], ne);
+        store_through(&p[2], ex);
+    }
+}
+
+// The level's counters summed over `blocks` entries, in every thread of the block.
+__device__ LevelOut dl_totals(const unsigned long long* part_slot, uint32_t blocks) {
+    __shared__ unsigned long long s_red[kDlThreads / kWave];
+    unsigned long long v[3] = {0, 0, 0};
+    for (uint32_t i = threadIdx.x; i < blocks; i += blockDim.x)
+        for (int j = 0; j < 3; ++j) v[j] += load_coherent(&part_slot[(size_t)i * kDlParts + j]);
+    LevelOut r;
+    r.found = block_sum(v[0], s_red);
+    r.next_edges = block_sum(v[1], s_red);
+    r.examined = block_sum(v[2], s_red);
+    return r;
+}
+
+// The decision step shared by both loops. Every thread of the calling block
+// keeps the same LoopState; wave 0 evaluates the learned policy. Returns
+// false when the traversal is over. `writer`: this block writes the trace.
+__device__ bool dl_decide(const DlDev& D, LoopState& st, const PolicyView& pv, uint32_t& prev_action, bool started,
+                          const unsigned long long* part_slot, uint32_t blocks, bool writer, uint64_t& t_prev,
+                          double tick_ms, const float* mlp_lds) {
+    __shared__ float s_y[3];
+    if (started) {
+        const LevelOut r = dl_totals(part_slot, blocks);
+        if (writer && threadIdx.x == 0 && D.trace && st.level < D.trace_cap) {
+            const uint64_t now = wall_clock64();
+            ggpu_level& L = D.trace[st.level];
+            if (prev_action == GGPU_ACTION_BU) L.edges = r.examined;
+            L.time_ms = (double)(now - t_prev) * tick_ms;
+        }
+        fold_level(st, r, pv, D.n);
+    }
+    if (st.size == 0) return false;
+    const bool learned = pv.mode == GGPU_POLICY_LEARNED && pv.mlp && pv.variant != GGPU_BFS_EDGE &&
+                         pv.variant != GGPU_BFS_BINNED;
+    if (learned) {
+        if (threadIdx.x < kWave) {
+            float f[5], y[3];
+            policy_features(st.size, st.total, st.unvisited_edges, st.level, pv.mean_degree, f);
+            mlp_forward_wave(mlp_lds, f, y);
+            if (threadIdx.x == 0)
+                for (int q = 0; q < 3; ++q) s_y[q] = y[q];
+        }
+        __syncthreads();
+    }
+    decide_action(st, pv, D.n, learned ? s_y : nullptr);
+    prev_action = st.action;
+    if (writer && threadIdx.x == 0) {
+        t_prev = wall_clock64();
+        if (D.trace && st.level < D.trace_cap) {
+            LevelOut none;
+            none.examined = st.total;
+            D.trace[st.level] = trace_record(st, none, 'G', 0.0);
+        }
+    }
+    if (learned) __syncthreads();  // s_y is rewritten at the next level
+    return true;
+}
+
+// --------------------------------------------------------------- kernels
+// Before either loop: dist = INF, vis and the frontier slots zero, the source seeded.
+__global__ void k_dl_init(DlDev D, uint32_t src) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < D.n; i += stride) D.dist[i] = GGPU_INF;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < D.nb; i += stride) {
+        const uint64_t w = (i == (src >> 6)) ? 1ull << (src & 63) : 0ull;
+        D.vis[i] = w;
+        D.fb[i] = w;
+        D.fb[D.nb + i] = 0;
+        D.fb[2 * (size_t)D.nb + i] = 0;
+    }
+}
+
+__global__ void k_dl_seed(DlDev D, uint32_t src) { D.dist[src] = 0; }
+
+__global__ void __launch_bounds__(kDlThreads) k_dl_persistent(DlDev D) {
+    __shared__ float s_mlp[GGPU_MLP_PARAMS];
+    LoopState st = D.ds->st;
+    const PolicyView pv = D.ds->pv;
+    const double tick_ms = D.ds->tick_ms;
+    if (pv.mlp)
+        for (uint32_t i = threadIdx.x; i < GGPU_MLP_PARAMS; i += blockDim.x) s_mlp[i] = pv.mlp[i];
+    __syncthreads();
+    uint32_t prev_action = 0;
+    uint64_t t_prev = 0;
+    const bool writer = blockIdx.x == 0;
+    bool started = false;
+    for (;;) {
+        // the counters of level L (still st.level before the fold) are in slot (L + 1) & 1
+        if (!dl_decide(D, st, pv, prev_action, started,
+                       D.part + (size_t)((st.level + 1) & 1) * gridDim.x * kDlParts, gridDim.x, writer, t_prev,
+                       tick_ms, s_mlp))
+            break;
+        started = true;
+        dl_level(D, st.level, st.action, D.part + (size_t)((st.level + 1) & 1) * gridDim.x * kDlParts);
+        if (!grid_barrier<kAcquireL1>(D.bar, D.abort)) return;
+    }
+    if (writer && threadIdx.x == 0) {
+        D.ds->st = st;
+        D.ds->levels = st.level;
+        D.ds->done = 1;
+    }
+}
+
+// Graph mode: one block of one wave advances the state...
+__global__ void __launch_bounds__(kWave) k_dl_decide(DlDev D, uint32_t level_blocks) {
+    __shared__ float s_mlp[GGPU_MLP_PARAMS];
+    __shared__ LoopState s_st;
+    DlState* ds = D.ds;
+    if (ds->done) return;
+    const PolicyView pv = ds->pv;
+    if (pv.mlp)
+        for (uint32_t i = threadIdx.x; i < GGPU_MLP_PARAMS; i += blockDim.x) s_mlp[i] = pv.mlp[i];
+    if (threadIdx.x == 0) s_st = ds->st;
+    __syncthreads();
+    LoopState st = s_st;
+    uint32_t prev_action = ds->prev_action;
+    uint64_t t_prev = ds->t_prev;
+    const bool started = ds->started != 0;
+    const bool more = dl_decide(D, st, pv, prev_action, started,
+                                D.part + (size_t)((st.level + 1) & 1) * level_blocks * kDlParts, level_blocks, true,
+                                t_prev, ds->tick_ms, s_mlp);
+    if (threadIdx.x == 0) {
+        ds->st = st;
+        ds->prev_action = prev_action;
+        ds->t_prev = t_prev;
+        ds->started = 1;
+        if (!more) {
+            ds->done = 1;
+            ds->levels = st.level;
+        }
+    }
+}
+
+// ...and the level kernel runs the chosen action on the whole grid.
+__global__ void __launch_bounds__(kDlLevelThreads) k_dl_level(DlDev D) {
+    const DlState* ds = D.ds;
+    if (ds->done) return;
+    const uint32_t level = ds->st.level, action = ds->st.action;
+    dl_level(D, level, action, D.part + (size_t)((level + 1) & 1) * gridDim.x * kDlParts);
+}
+
+// ------------------------------------------------------------------- host
+struct DlResources {
+    DBuf<uint64_t> vis, fb;
+    DBuf<unsigned long long> part;
+    DBuf<DlState> ds;
+    DBuf<ggpu_level> trace;
+    DBuf<uint8_t> actions;
+    DBuf<uint32_t> ctl;
+    uint32_t persist_grid = 0, level_grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t graph_k = 0;
+    DlDev graph_dev{};  // the arguments the graph was captured with
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~DlResources() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+static void dl_free(void* p) { delete static_cast<DlResources*>(p); }
+
+}  // namespace ggpu
 
 namespace ggpu {
 
 void BfsEngine::device_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, std::vector<ggpu_level>* trace) {
-    (void)variant; (void)src; (void)o; (void)trace;
-    throw ArgError("device-driven loops are not built yet");
+    if (variant == GGPU_BFS_HYBRID) throw ArgError("hybrid runs with the host loop");
+    if (o.loop == GGPU_LOOP_GRAPH && (o.graph_levels == 0 || o.graph_levels > 4096))
+        throw ArgError("graph_levels must be in 1..4096");
+    hipStream_t s = ctx->stream;
+    if (!dl_state) {
+        dl_state = new DlResources();
+        dl_free_fn = dl_free;
+    }
+    DlResources& R = *static_cast<DlResources*>(dl_state);
+    if (!R.ev[0]) {
+        GGPU_HIP_TRY(hipEventCreate(&R.ev[0]));
+        GGPU_HIP_TRY(hipEventCreate(&R.ev[1]));
+    }
+    if (!R.persist_grid) {
+        int per_cu = 0;
+        GGPU_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_dl_persistent, kDlThreads, 0));
+        if (per_cu < 1) throw ArgError("persistent BFS kernel does not fit on a compute unit", GGPU_EDEVICE);
+        R.persist_grid = (uint32_t)ctx->cus;
+        R.level_grid = (uint32_t)ctx->cus * 4;
+    }
+    const uint32_t nb = (n + 63) / 64;
+    R.vis.alloc(nb + 1);
+    R.fb.alloc(3ull * nb + 1);
+    R.part.alloc(2ull * std::max(R.persist_grid, R.level_grid) * kDlParts);
+    R.ds.alloc(1);
+    R.ctl.alloc(kBarWords + 8);
+    const size_t tcap = trace ? (size_t)n + 2 : 0;
+    if (trace) R.trace.alloc(tcap);
+    const float* mlp = (o.policy == GGPU_POLICY_LEARNED && ctx->policy_set[GGPU_POLICY_SINGLE])
+                           ? ctx->policy_dev[GGPU_POLICY_SINGLE] : nullptr;
+    if (o.policy == GGPU_POLICY_SCRIPTED) {
+        R.actions.alloc(o.n_actions);
+        GGPU_HIP_TRY(hipMemcpyAsync(R.actions.p, o.actions, o.n_actions, hipMemcpyHostToDevice, s));
+    }
+    DlState h{};
+    h.st = loop_start(src, variant, o);
+    h.pv = policy_view(variant, o, mlp);
+    h.pv.actions = o.policy == GGPU_POLICY_SCRIPTED ? R.actions.p : nullptr;
+    int clock_khz = 0;
+    GGPU_HIP_TRY(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device));
+    h.tick_ms = clock_khz > 0 ? 1.0 / clock_khz : 0.0;
+    GGPU_HIP_TRY(hipMemcpyAsync(R.ds.p, &h, sizeof h, hipMemcpyHostToDevice, s));
+    GGPU_HIP_TRY(hipMemsetAsync(R.ctl.p, 0, (kBarWords + 8) * sizeof(uint32_t), s));
+
+    DlDev D{};
+    D.rp = rp.p;
+    D.ci = ci.p;
+    D.dist = dist.p;
+    D.vis = R.vis.p;
+    D.fb = R.fb.p;
+    D.part = R.part.p;
+    D.heavy = heavy.p;
+    D.ds = R.ds.p;
+    D.trace = trace ? R.trace.p : nullptr;
+    D.bar = R.ctl.p;
+    D.abort = R.ctl.p + kBarWords;
+    D.n_heavy = n_heavy;
+    D.n = n;
+    D.nb = nb;
+    D.trace_cap = (uint32_t)std::min<size_t>(tcap, 0xFFFFFFFFu);
+    D.m = m;
+
+    GGPU_HIP_TRY(hipEventRecord(R.ev[0], s));
+    k_dl_init<<<blocks_for(std::max(n, nb), 256, grid_cap), 256, 0, s>>>(D, src);
+    k_dl_seed<<<1, 1, 0, s>>>(D, src);
+    GGPU_HIP_TRY(hipGetLastError());
+    DlState out{};
+    if (o.loop == GGPU_LOOP_PERSISTENT) {
+        void* args[] = {&D};
+        GGPU_HIP_TRY(hipLaunchCooperativeKernel((const void*)k_dl_persistent, dim3(R.persist_grid), dim3(kDlThreads),
+                                                args, 0, s));
+        GGPU_HIP_TRY(hipEventRecord(R.ev[1], s));
+        uint32_t abort = 0;
+        GGPU_HIP_TRY(hipMemcpyAsync(&abort, D.abort, 4, hipMemcpyDeviceToHost, s));
+        GGPU_HIP_TRY(hipMemcpyAsync(&out, R.ds.p, sizeof out, hipMemcpyDeviceToHost, s));
+        GGPU_HIP_TRY(hipStreamSynchronize(s));
+        if (abort) throw ArgError("persistent BFS: grid barrier timed out", GGPU_ETIMEOUT);
+    } else {
+        // a graph of K levels; the captured kernels read everything else from device memory
+        if (!R.exec || R.graph_k != o.graph_levels || std::memcmp(&R.graph_dev, &D, sizeof D) != 0) {
+            if (R.exec) GGPU_HIP_TRY(hipGraphExecDestroy(R.exec));
+            if (R.graph) GGPU_HIP_TRY(hipGraphDestroy(R.graph));
+            R.exec = nullptr;
+            R.graph = nullptr;
+            GGPU_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (uint32_t l = 0; l < o.graph_levels; ++l) {
+                k_dl_decide<<<1, kWave, 0, s>>>(D, R.level_grid);
+                k_dl_level<<<R.level_grid, kDlLevelThreads, 0, s>>>(D);
+            }
+            GGPU_HIP_TRY(hipStreamEndCapture(s, &R.graph));
+            GGPU_HIP_TRY(hipGraphInstantiate(&R.exec, R.graph, nullptr, nullptr, 0));
+            R.graph_k = o.graph_levels;
+            R.graph_dev = D;
+        }
+        uint32_t* done = (uint32_t*)ctx_pinned(ctx, 4096);
+        for (uint64_t launches = 0;; ++launches) {
+            if (launches > (uint64_t)n / o.graph_levels + 2) throw ArgError("graph loop did not finish", GGPU_ESTATE);
+            GGPU_HIP_TRY(hipGraphLaunch(R.exec, s));
+            GGPU_HIP_TRY(hipMemcpyAsync(done, &R.ds.p->done, 4, hipMemcpyDeviceToHost, s));
+            GGPU_HIP_TRY(hipStreamSynchronize(s));
+            if (*done) break;
+        }
+        GGPU_HIP_TRY(hipEventRecord(R.ev[1], s));
+        GGPU_HIP_TRY(hipMemcpyAsync(&out, R.ds.p, sizeof out, hipMemcpyDeviceToHost, s));
+        GGPU_HIP_TRY(hipStreamSynchronize(s));
+    }
+    float ms = 0;
+    GGPU_HIP_TRY(hipEventElapsedTime(&ms, R.ev[0], R.ev[1]));
+    last_gpu_ms = ms;
+    last_levels = out.levels;
+    if (trace) {
+        trace->resize(out.levels);
+        if (out.levels)
+            GGPU_HIP_TRY(hipMemcpy(trace->data(), R.trace.p, out.levels * sizeof(ggpu_level), hipMemcpyDeviceToHost));
+    }
 }
 
 }  // namespace ggpu

@@ -7,8 +7,16 @@
 //   * between two barriers, data that another block reads after the barrier
 //     is written only with device-scope atomics or agent-scope (write-through)
 //     stores -- both reach the device coherence point;
-//   * every block invalidates its caches when it leaves the barrier
-//     (agent-scope acquire), so its plain loads see that data.
+//   * on leaving, a block either
+//       - kAcquireL2: invalidates its caches at agent scope (L1 and the
+//         XCD's L2), after which plain loads see that data; or
+//       - kAcquireL1: invalidates only its L1 and reads data other blocks
+//         wrote with coherent (agent-scope) loads; the L2 then keeps the
+//         read-only graph arrays across barriers. Plain loads of such data
+//         can return stale L2 lines on MI355X (a kernel reading per-block
+//         counters that way diverged between blocks).
+// The agent-scope invalidate costs every block's L2 contents each barrier
+// (tens of us per level when levels are small).
 // Blocks arrive on kBarGroups counters (blockIdx % kBarGroups), the last of a
 // group on a top counter, the last of those bumps a generation word: about
 // 5.4 us on 256 blocks of 512 threads instead of 17 us for one counter.
@@ -41,8 +49,11 @@ __device__ __forceinline__ T load_coherent(const T* p) {
     return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+enum AcquireKind { kAcquireL2 = 0, kAcquireL1 = 1 };
+
 // Called by every thread of every block. Returns false once any block has
 // given up waiting; the caller then returns from the kernel.
+template <AcquireKind kAcquire = kAcquireL2>
 __device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t* abort, uint64_t spin_limit = kBarSpinLimit) {
     __builtin_amdgcn_s_waitcnt(0);  // this thread's stores and atomics have completed
     __syncthreads();
@@ -52,12 +63,13 @@ __device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t* abort, uin
         const uint32_t g = blockIdx.x % kBarGroups;
         const uint32_t gsize = G / kBarGroups + (g < G % kBarGroups ? 1u : 0u);
         const uint32_t ngroups = G < kBarGroups ? G : kBarGroups;
+        // The counters only grow (zeroed per launch; 2^32 arrivals are out of
+        // reach): the last arriver of a round is the one that completes a
+        // multiple of the group size. Resetting them instead would race with
+        // a fast block's arrival at the next barrier.
         bool released = false;
-        if (atomicAdd(&bar[g * kBarStride], 1u) == gsize - 1) {
-            store_through(&bar[g * kBarStride], 0u);
-            if (atomicAdd(&bar[kBarTop], 1u) == ngroups - 1) {
-                store_through(&bar[kBarTop], 0u);
-                __builtin_amdgcn_s_waitcnt(0);
+        if ((atomicAdd(&bar[g * kBarStride], 1u) + 1) % gsize == 0) {
+            if ((atomicAdd(&bar[kBarTop], 1u) + 1) % ngroups == 0) {
                 __hip_atomic_fetch_add(&bar[kBarGen], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 released = true;
             }
@@ -72,7 +84,11 @@ __device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t* abort, uin
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (kAcquire == kAcquireL2) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        else asm volatile("buffer_inv sc0" ::: "memory");  // the CU's L1 only
     }
     __syncthreads();
     return load_coherent(abort) == 0u;

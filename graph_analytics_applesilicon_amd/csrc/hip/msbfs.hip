@@ -532,10 +532,8 @@ __device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool
 struct MsPolicyState {
     uint32_t bottom_up = 0;
     uint64_t prev_size = 0;
-    unsigned long long cum_pairs = 0;
 };
 
-// Out of line: the MLP's hidden layers stay out of the traversal's registers.
 __device__ __forceinline__ void ms_features(const MsDev& D, uint32_t level, const MsAcc& t, uint64_t pairs_u,
                                             float* f) {
     // frontier vertices, frontier and unreached edge counts per source (pair-edges / k)
@@ -544,17 +542,15 @@ __device__ __forceinline__ void ms_features(const MsDev& D, uint32_t level, cons
                     D.n ? (double)D.m / (double)D.n : 0.0, f);
 }
 
-__device__ __noinline__ uint32_t ms_choose(const MsDev& D, const float* mlp, uint32_t level, const MsAcc& t,
-                                           uint64_t pairs_u, MsPolicyState& ps) {
+// y: the MLP's predictions for this level (learned policy with parameters).
+__device__ uint32_t ms_choose(const MsDev& D, const float (&y)[3], uint32_t level, const MsAcc& t, uint64_t pairs_u,
+                              MsPolicyState& ps) {
     uint32_t act;
     if (D.policy == GGPU_POLICY_SCRIPTED) {
         act = D.actions[level % D.n_actions];
         if (act > GGPU_ACTION_BU) act = GGPU_ACTION_BU;
         ps.bottom_up = act == GGPU_ACTION_BU;
     } else if (D.policy == GGPU_POLICY_LEARNED && D.mlp) {
-        float f[5], y[3];
-        ms_features(D, level, t, pairs_u, f);
-        mlp_forward1(mlp, f, y);
         act = 0;
         for (uint32_t q = 1; q < 3; ++q)
             if (y[q] < y[act]) act = q;
@@ -590,11 +586,19 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
     uint32_t base = 0, level = 0;
     const unsigned long long total_pairs = (unsigned long long)D.k * D.m;
     uint64_t t_prev = 0;
+    unsigned long long cum_pairs = 0;  // identical in every thread: t is the same everywhere
     for (;; ++level) {
         const MsAcc t = ms_totals(D, level & 1);
+        cum_pairs += t.pairs;
+        const uint64_t pairs_u = total_pairs > cum_pairs ? total_pairs - cum_pairs : 0;
+        float y[3] = {0.f, 0.f, 0.f};
+        if (D.policy == GGPU_POLICY_LEARNED && D.mlp && threadIdx.x < kWave && t.size) {
+            // the level boundary's inference: wave 0 evaluates the MLP on this level's features
+            float f[5];
+            ms_features(D, level, t, pairs_u, f);
+            mlp_forward_wave(s_mlp, f, y);
+        }
         if (threadIdx.x == 0) {
-            ps.cum_pairs += t.pairs;
-            const uint64_t pairs_u = total_pairs > ps.cum_pairs ? total_pairs - ps.cum_pairs : 0;
             if (blockIdx.x == 0) {
                 const uint64_t now = wall_clock64();
                 if (level > 0 && level - 1 < D.trace_cap) {
@@ -605,7 +609,7 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
             }
             uint32_t act = 0xFFFFFFFFu;
             if (t.size) {
-                act = ms_choose(D, s_mlp, level, t, pairs_u, ps);
+                act = ms_choose(D, y, level, t, pairs_u, ps);
                 counts[act]++;
                 if (blockIdx.x == 0 && level < D.trace_cap) {
                     ggpu_level L{};
