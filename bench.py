@@ -87,7 +87,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build runs on a single "
-              "MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+              "MI355X; batches of sources and PPR seeds shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     import graph_analytics_applesilicon_amd as G

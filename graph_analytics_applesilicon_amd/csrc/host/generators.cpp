@@ -6,7 +6,7 @@
 //       SplitMix64 so edges are generated in parallel yet equal the serial
 //       stream of the reference.
 //   gen_grid with keep < 1 and the *_c* workload graphs are new: they are the
-//       inputs the benchmark configurations name (recipes in DESIGN.md).
+//       inputs the benchmark configurations name (recipes in gen_named below).
 #include "graph.hpp"
 
 #include <omp.h>

@@ -60,7 +60,10 @@ int ggpu_graph_largest_component(const ggpu_graph* g, ggpu_graph** out, uint32_t
 /* ---- generators (src/graph/generators.cpp:8-88, extended) ----
  * name: path1000 star100k complete200 two_components empty1000 grid100x100
  *       rmat12 rmat12_directed (the reference suite) and the workload graphs
- *       rmat16_c1, rmat22_c2, grid_c3, rmat23_c5 (see DESIGN.md for their recipes). */
+ *       rmat16_c1 = permute(rmat(16, 16, .57, .19, .19, seed 1), random_perm(seed 1)),
+ *       rmat22_c2 = permute(rmat(22, 8, .57, .19, .19, seed 2), random_perm(seed 2)),
+ *       grid_c3   = largest component of grid(1024, 2048, keep 0.7, seed 3),
+ *       rmat23_c5 = rmat(23, 16, .60, .17, .17, undirected, seed 5), ids not permuted. */
 int ggpu_gen_named(const char* name, ggpu_graph** out);
 int ggpu_gen_rmat(uint32_t scale, uint32_t edgefactor, double a, double b, double c, int directed,
                   uint64_t seed, ggpu_graph** out);
