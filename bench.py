@@ -293,21 +293,24 @@ def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
     log(f"FP64 converged oracle (the reference's pagerankOmp, tol 1e-10): {time.time() - t0:.1f}s")
     res = {}
     bytes_sweep = 4 * (n + 1) + 8 * m + 4 * n
-    # per-sweep time of the fused Jacobi sweep in each vertex order (20 sweeps, no host sync)
-    sweeps = {}
-    for order, name in ((G.PR_ORDER_GIVEN, "given"), (G.PR_ORDER_DEGREE, "degree")):
-        fo = G.pr_options(iters=20, order=order)
-        pr.run("gpu_pull_adaptive", fo)
-        r = pr.run("gpu_pull_adaptive", fo)
-        sweeps[name] = r.gpu_ms / 20
-        log(f"  sweep ({name} order): {sweeps[name] * 1e3:.1f} us")
+    # per-sweep time of the fused Jacobi sweep per layout and vertex order (20 sweeps, no host sync)
+    sweeps, combos = {}, {}
+    for layout, lname in ((G.PR_LAYOUT_STREAM, "stream"), (G.PR_LAYOUT_HOT, "hot")):
+        for order, oname in ((G.PR_ORDER_GIVEN, "given"), (G.PR_ORDER_DEGREE, "degree")):
+            name = f"{lname}/{oname}"
+            fo = G.pr_options(iters=20, order=order, layout=layout)
+            pr.run("gpu_pull_adaptive", fo)
+            r = pr.run("gpu_pull_adaptive", fo)
+            sweeps[name] = r.gpu_ms / 20
+            combos[name] = (layout, order)
+            log(f"  sweep ({name}): {sweeps[name] * 1e3:.1f} us")
     best_name = min(sweeps, key=sweeps.get)
-    order = G.PR_ORDER_DEGREE if best_name == "degree" else G.PR_ORDER_GIVEN
+    layout, order = combos[best_name]
     res["sweep_ms"] = sweeps
     # Jacobi, certified stop: ||r_k - r*||_1 <= d/(1-d) * delta_k  =>  delta_k < 1e-4 * (1-d)/d
     tol_cert = 1e-4 * 0.15 / 0.85
-    for variant, opts, key in (("gpu_pull_adaptive", G.pr_options(tol=tol_cert, iters=1000, order=order),
-                                "jacobi_certified"),
+    for variant, opts, key in (("gpu_pull_adaptive", G.pr_options(tol=tol_cert, iters=1000, order=order,
+                                                                  layout=layout), "jacobi_certified"),
                                ("gpu_pull_gs", G.pr_options(tol=1e-6, iters=1000, order=order), "gs_tol_1e-6")):
         pr.run(variant, opts)  # warm-up (builds the order)
         t1 = time.perf_counter()
@@ -320,9 +323,10 @@ def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
         res[key] = ent
     per_sweep_ms = sweeps[best_name]
     ach = bytes_sweep / (per_sweep_ms / 1e3) / 1e9
-    res["roofline"] = {"kernel": f"k_pr_sweep (gpu_pull_adaptive, Jacobi, stream layout, {best_name} order)",
+    kname = "k_pr_sweep_hot" if layout == G.PR_LAYOUT_HOT else "k_pr_sweep"
+    res["roofline"] = {"kernel": f"{kname} (gpu_pull_adaptive, Jacobi, {best_name} layout/order)",
                        "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("k_pr_sweep", g_name),
+                       "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(kname, g_name),
                        "bytes_per_launch": bytes_sweep,
                        "avg_launch_ms": per_sweep_ms}
     log(f"pagerank: certified Jacobi {res['jacobi_certified']['ms']:.2f} ms "

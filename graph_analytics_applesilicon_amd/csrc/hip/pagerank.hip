@@ -35,6 +35,7 @@ constexpr uint32_t kPartialCap = 1u << 17;        // FP64 partial slots per quan
 
 // ------------------------------------------------------------------ kernels
 __global__ void k_pr_sweep(PrSweep P) { pr_sweep_body(P); }
+__global__ void __launch_bounds__(kHotThreads) k_pr_sweep_hot(PrSweep P) { pr_hot_sweep_body(P); }
 
 __global__ void k_pr_longfix(PrSweep P) { pr_longfix_body(P); }
 __global__ void k_pr_reduce(const double* partial, uint32_t cap, uint32_t count, PrAcc* out) {
@@ -105,6 +106,7 @@ struct Space {
     std::vector<std::pair<uint32_t, uint32_t>> ranges;  // GS: consecutive update ranges
     DBuf<uint32_t> rp, ci, odd;
     std::vector<std::unique_ptr<StreamLayout>> stream;  // one per range
+    std::unique_ptr<StreamLayout> wstream;              // HOT layout: wave-sized items over all rows
     DBuf<uint32_t> bins;
     uint32_t bin_count[3] = {0, 0, 0};
     uint32_t bin_small = 0, bin_large = 0;
@@ -182,6 +184,7 @@ struct PrEngine {
         S.id_of.resize(n);
         for (uint32_t v = 0; v < n; ++v) S.id_of[v] = perm[base.id_of.empty() ? v : base.id_of[v]];
         S.stream.clear();
+        S.wstream.reset();
         S.bins_ready = false;
     }
 
@@ -203,51 +206,59 @@ struct PrEngine {
         throw ArgError("space not prepared");
     }
 
+    // Row blocks of <= nnz_cap entries and <= rows_cap rows over rows [ra, rb);
+    // rows longer than nnz_cap become chunks of <= chunk entries.
+    static std::unique_ptr<StreamLayout> make_stream(const Space& S, uint32_t ra, uint32_t rb, uint32_t nnz_cap,
+                                                     uint32_t rows_cap, uint32_t chunk) {
+        auto L = std::make_unique<StreamLayout>();
+        L->r_begin = ra;
+        L->r_end = rb;
+        std::vector<uint2> sitems, lrows;
+        std::vector<uint4> litems;
+        uint32_t cur0 = ra, rows = 0, nnz = 0;
+        auto close = [&](uint32_t end) {
+            if (rows) sitems.push_back(make_uint2(cur0, end));
+            rows = 0;
+            nnz = 0;
+        };
+        for (uint32_t v = ra; v < rb; ++v) {
+            const uint32_t dg = S.in.deg(v);
+            if (dg > nnz_cap) {
+                close(v);
+                const uint32_t first = (uint32_t)litems.size();
+                for (uint32_t x = S.in.rp[v]; x < S.in.rp[v + 1]; x += chunk)
+                    litems.push_back(make_uint4(v, x, std::min(x + chunk, S.in.rp[v + 1]), (uint32_t)lrows.size()));
+                lrows.push_back(make_uint2(first, (uint32_t)litems.size() - first));
+                continue;
+            }
+            if (rows && (nnz + dg > nnz_cap || rows == rows_cap)) close(v);
+            if (!rows) cur0 = v;
+            ++rows;
+            nnz += dg;
+        }
+        close(rb);
+        L->ns = (uint32_t)sitems.size();
+        L->nl = (uint32_t)litems.size();
+        L->nlr = (uint32_t)lrows.size();
+        L->sitems.alloc(L->ns + 1);
+        L->litems.alloc(L->nl + 1);
+        L->lrows.alloc(L->nlr + 1);
+        L->lpart.alloc(L->nl + 1);
+        for (const uint2& lr : lrows) L->multi |= lr.y > 1;
+        if (L->ns) GGPU_HIP_TRY(hipMemcpy(L->sitems.p, sitems.data(), L->ns * sizeof(uint2), hipMemcpyHostToDevice));
+        if (L->nl) GGPU_HIP_TRY(hipMemcpy(L->litems.p, litems.data(), L->nl * sizeof(uint4), hipMemcpyHostToDevice));
+        if (L->nlr) GGPU_HIP_TRY(hipMemcpy(L->lrows.p, lrows.data(), L->nlr * sizeof(uint2), hipMemcpyHostToDevice));
+        return L;
+    }
+
     void build_stream(Space& S) {
         if (S.stream.size() == S.ranges.size()) return;
         S.stream.clear();
-        for (auto [ra, rb] : S.ranges) {
-            auto L = std::make_unique<StreamLayout>();
-            L->r_begin = ra;
-            L->r_end = rb;
-            std::vector<uint2> sitems, lrows;
-            std::vector<uint4> litems;
-            uint32_t cur0 = ra, rows = 0, nnz = 0;
-            auto close = [&](uint32_t end) {
-                if (rows) sitems.push_back(make_uint2(cur0, end));
-                rows = 0;
-                nnz = 0;
-            };
-            for (uint32_t v = ra; v < rb; ++v) {
-                const uint32_t dg = S.in.deg(v);
-                if (dg > kStreamNnz) {
-                    close(v);
-                    const uint32_t first = (uint32_t)litems.size();
-                    for (uint32_t x = S.in.rp[v]; x < S.in.rp[v + 1]; x += kLongChunk)
-                        litems.push_back(
-                            make_uint4(v, x, std::min(x + kLongChunk, S.in.rp[v + 1]), (uint32_t)lrows.size()));
-                    lrows.push_back(make_uint2(first, (uint32_t)litems.size() - first));
-                    continue;
-                }
-                if (rows && (nnz + dg > kStreamNnz || rows == kStreamRows)) close(v);
-                if (!rows) cur0 = v;
-                ++rows;
-                nnz += dg;
-            }
-            close(rb);
-            L->ns = (uint32_t)sitems.size();
-            L->nl = (uint32_t)litems.size();
-            L->nlr = (uint32_t)lrows.size();
-            L->sitems.alloc(L->ns + 1);
-            L->litems.alloc(L->nl + 1);
-            L->lrows.alloc(L->nlr + 1);
-            L->lpart.alloc(L->nl + 1);
-            for (const uint2& lr : lrows) L->multi |= lr.y > 1;
-            if (L->ns) GGPU_HIP_TRY(hipMemcpy(L->sitems.p, sitems.data(), L->ns * sizeof(uint2), hipMemcpyHostToDevice));
-            if (L->nl) GGPU_HIP_TRY(hipMemcpy(L->litems.p, litems.data(), L->nl * sizeof(uint4), hipMemcpyHostToDevice));
-            if (L->nlr) GGPU_HIP_TRY(hipMemcpy(L->lrows.p, lrows.data(), L->nlr * sizeof(uint2), hipMemcpyHostToDevice));
-            S.stream.push_back(std::move(L));
-        }
+        for (auto [ra, rb] : S.ranges) S.stream.push_back(make_stream(S, ra, rb, kStreamNnz, kStreamRows, kLongChunk));
+    }
+
+    void build_wave_stream(Space& S) {
+        if (!S.wstream) S.wstream = make_stream(S, 0, n, kWaveNnz, kWaveNnz, kWaveNnz);
     }
 
     void build_bins(Space& S, uint32_t small_max, uint32_t large_min) {
@@ -342,7 +353,7 @@ struct PrEngine {
     }
 
     // Stream-layout launch geometry: long-chunk blocks first, then row blocks.
-    uint32_t use_stream(PrSweep& P, StreamLayout& L) {
+    uint32_t use_stream(PrSweep& P, StreamLayout& L, uint32_t mode = 1) {
         PrStreamLayout& Y = P.stream;
         Y.sitems = L.sitems.p;
         Y.n_sitems = L.ns;
@@ -351,8 +362,12 @@ struct PrEngine {
         Y.lrows = L.lrows.p;
         Y.lpart = L.lpart.p;
         Y.n_lrows = L.nlr;
-        P.use_stream = 1;
+        P.use_stream = mode;
         P.stream_multi = L.multi;
+        if (mode == 2) {  // one workgroup per CU, kHotWaves waves each
+            const uint64_t want = ((uint64_t)L.ns + L.nl + kHotWaves - 1) / kHotWaves;
+            return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)std::max(ctx->cus, 1)));
+        }
         return std::max<uint32_t>(1, std::min<uint32_t>(L.ns + L.nl, kStreamBlocks));
     }
 
@@ -373,7 +388,8 @@ struct PrEngine {
         if (npart + grid > kPartialCap) throw ArgError("too many blocks in one sweep");
         P.partial_base = npart;
         npart += grid;
-        k_pr_sweep<<<grid, tg, 0, ctx->stream>>>(P);
+        if (P.use_stream == 2) k_pr_sweep_hot<<<grid, kHotThreads, 0, ctx->stream>>>(P);
+        else k_pr_sweep<<<grid, tg, 0, ctx->stream>>>(P);
         GGPU_HIP_TRY(hipGetLastError());
         if (P.use_stream && P.stream_multi) {
             const uint32_t g = blocks(P.stream.n_lrows, 256 / kWave);
@@ -397,7 +413,11 @@ struct PrEngine {
         uint32_t grid = 0;
         if (variant == GGPU_PR_SCALAR) return add_section(P, nullptr, 0, n, SEC_LANE, 1, tg, grid);
         if (variant == GGPU_PR_VECTOR) return add_section(P, nullptr, 0, n, SEC_GROUP, kWave, tg, grid);
-        if (o.layout == GGPU_PR_LAYOUT_STREAM) {
+        if (o.layout == GGPU_PR_LAYOUT_HOT && P.contrib_in != P.contrib_out) {
+            build_wave_stream(S);
+            return use_stream(P, *S.wstream, 2);
+        }
+        if (o.layout != GGPU_PR_LAYOUT_BINS) {  // STREAM, and in-place HOT
             build_stream(S);
             return use_stream(P, *S.stream[0]);
         }
@@ -438,12 +458,13 @@ struct PrEngine {
         const uint32_t tg = o.tg;
         if (variant == GGPU_PR_GS) {
             // ranges in order: colour sets, or natural order then deferred partners
-            if (o.layout == GGPU_PR_LAYOUT_STREAM) build_stream(S);
+            const bool rowblocks = o.layout != GGPU_PR_LAYOUT_BINS;  // HOT needs fixed contributions
+            if (rowblocks) build_stream(S);
             for (size_t i = 0; i < S.ranges.size(); ++i) {
                 PrSweep P = fresh(i == 0);
                 const auto [a, b] = S.ranges[i];
                 uint32_t grid;
-                if (o.layout == GGPU_PR_LAYOUT_STREAM) grid = use_stream(P, *S.stream[i]);
+                if (rowblocks) grid = use_stream(P, *S.stream[i]);
                 else if (gs_colored) grid = add_section(P, nullptr, a, b - a, SEC_LANE, 1, tg, 0);
                 else grid = add_section(P, nullptr, a, b - a, SEC_GROUP, kWave, tg, 0);
                 launch(P, grid, tg);
@@ -467,7 +488,7 @@ struct PrEngine {
         const uint32_t L = o.medium_lanes;
         if (L == 0 || L > 64 || (L & (L - 1))) throw ArgError("medium_lanes must be a power of two in [1, 64]");
         if (o.iters < 0) throw ArgError("iters must be >= 0");
-        if (o.layout > GGPU_PR_LAYOUT_BINS) throw ArgError("unknown PageRank layout");
+        if (o.layout > GGPU_PR_LAYOUT_HOT) throw ArgError("unknown PageRank layout");
         if (o.order > GGPU_PR_ORDER_DEGREE) throw ArgError("unknown PageRank vertex order");
         const int base_id = o.order == GGPU_PR_ORDER_DEGREE ? SP_DEGREE : SP_GIVEN;
         int sid = base_id;
@@ -562,7 +583,9 @@ struct PrEngine {
     void autotune(int iters, ggpu_pr_opts& o) {
         struct Cand { uint32_t layout, order, sm, lg, L; };
         std::vector<Cand> cands = {{GGPU_PR_LAYOUT_STREAM, GGPU_PR_ORDER_GIVEN, 0, 0, 64},
-                                   {GGPU_PR_LAYOUT_STREAM, GGPU_PR_ORDER_DEGREE, 0, 0, 64}};
+                                   {GGPU_PR_LAYOUT_STREAM, GGPU_PR_ORDER_DEGREE, 0, 0, 64},
+                                   {GGPU_PR_LAYOUT_HOT, GGPU_PR_ORDER_GIVEN, 0, 0, 64},
+                                   {GGPU_PR_LAYOUT_HOT, GGPU_PR_ORDER_DEGREE, 0, 0, 64}};
         for (uint32_t order : {(uint32_t)GGPU_PR_ORDER_GIVEN, (uint32_t)GGPU_PR_ORDER_DEGREE})
             for (uint32_t sm : {4u, 16u})
                 for (uint32_t lg : {256u, 4096u})

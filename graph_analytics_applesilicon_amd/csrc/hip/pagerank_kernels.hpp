@@ -69,7 +69,7 @@ struct PrSweep {
     PrSection sec[kMaxSections];
     uint32_t nsec;
     PrStreamLayout stream;    // used when use_stream != 0 (instead of the sections)
-    uint32_t use_stream;
+    uint32_t use_stream;      // 1: row blocks (k_pr_sweep), 2: HOT layout (k_pr_sweep_hot)
     uint32_t stream_multi;    // some long row spans several chunks (needs pr_longfix_body)
 };
 
@@ -238,6 +238,105 @@ __device__ __forceinline__ void pr_stream_body(const PrSweep& P, const PrStreamL
     }
 }
 
+// ------------------------------------------------------- hot-cached layout
+// gpu_pull_adaptive's HOT layout (Jacobi sweeps only: the cache is filled once
+// per sweep). One kHotThreads workgroup per CU runs the whole sweep. It first
+// copies contributions [0, min(n, kHotCap)) into LDS; in the degree-ordered
+// numbering those are the most gathered ones (on a skewed RMAT graph the top
+// 30K of 8.4M vertices are the sources of ~45% of the arcs), so those gathers
+// never enter the texture / L1 / L2 path. The work is wave-sized: row blocks
+// of <= kWaveNnz in-list entries and long-row chunks of kWaveNnz entries,
+// taken by the waves in turn, so no wave waits for another inside the sweep.
+constexpr uint32_t kHotThreads = 1024;
+constexpr uint32_t kHotWaves = kHotThreads / kWave;
+constexpr uint32_t kWaveNnz = 512;
+constexpr uint32_t kHotCap = 30720;  // 120 KiB of LDS, + 32 KiB of per-wave staging
+
+// Orders a wave's LDS writes before its later LDS reads by other lanes (a
+// wave's LDS operations execute in order; this keeps the compiler's too).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// gather_batched for one wave, with contributions below `hot` read from LDS.
+template <class Sink>
+__device__ __forceinline__ void gather_wave(const PrSweep& P, const float* s_hot, uint32_t hot, uint32_t x0,
+                                            uint32_t cnt, Sink sink) {
+    const uint32_t lane = lane_id();
+    for (uint32_t base = 0; base < cnt; base += kGatherBatch * kWave) {
+        uint32_t u[kGatherBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            const uint32_t i = base + j * kWave + lane;
+            u[j] = i < cnt ? __builtin_nontemporal_load(&P.ci[x0 + i]) : 0u;
+        }
+        float v[kGatherBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            const uint32_t i = base + j * kWave + lane;
+            float x = 0.0f;
+            if (i < cnt) {
+                if (u[j] < hot) x = s_hot[u[j]];
+                else x = P.contrib_in[u[j]];
+            }
+            v[j] = x;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            const uint32_t i = base + j * kWave + lane;
+            if (i < cnt) sink(i, v[j]);
+        }
+    }
+}
+
+__device__ __forceinline__ void pr_hot_body(const PrSweep& P, float add, float d, float inv_n, PrPart& part) {
+    __shared__ float s_hot[kHotCap];
+    __shared__ float s_stage[kHotWaves][kWaveNnz];
+    const PrStreamLayout& Lyt = P.stream;
+    const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
+    const uint32_t hot = P.n < kHotCap ? P.n : kHotCap;
+    for (uint32_t i = threadIdx.x; i < hot; i += blockDim.x) s_hot[i] = P.contrib_in[i];
+    __syncthreads();
+    float* st = s_stage[w];
+    const uint32_t n_items = Lyt.n_litems + Lyt.n_sitems;
+    for (uint32_t it = blockIdx.x * kHotWaves + w; it < n_items; it += gridDim.x * kHotWaves) {
+        if (it < Lyt.n_litems) {
+            const uint4 li = Lyt.litems[it];
+            Kahan k;
+            gather_wave(P, s_hot, hot, li.y, li.z - li.y, [&](uint32_t, float x) { k.add(x); });
+            float hi = k.s, lo = -k.c;
+            comp_reduce(hi, lo, kWave);
+            if (lane == 0) {
+                if (Lyt.lrows[li.w].y == 1) pr_store(P, li.x, add + d * (hi + lo), part, inv_n);
+                else Lyt.lpart[it] = make_float2(hi, lo);  // combined by pr_longfix_body
+            }
+            continue;
+        }
+        const uint2 si = Lyt.sitems[it - Lyt.n_litems];
+        const uint32_t r0 = si.x, r1 = si.y;
+        const uint32_t x0 = P.rp[r0], x1 = P.rp[r1];
+        gather_wave(P, s_hot, hot, x0, x1 - x0, [&](uint32_t i, float x) { st[i] = x; });
+        wave_lds_sync();
+        const uint32_t rows = r1 - r0;
+        uint32_t G = rows >= kWave ? 1u : kWave / rows;
+        G = G >= 64 ? 64 : (G >= 32 ? 32 : (G >= 16 ? 16 : (G >= 8 ? 8 : (G >= 4 ? 4 : (G >= 2 ? 2 : 1)))));
+        const uint32_t per_round = kWave / G, rr = lane / G, gl = lane % G;
+        for (uint32_t rb = 0; rb < rows; rb += per_round) {
+            const uint32_t r = r0 + rb + rr;
+            Kahan k;
+            if (r < r1) {
+                const uint32_t a = P.rp[r] - x0, b = P.rp[r + 1] - x0;
+                for (uint32_t x = a + gl; x < b; x += G) k.add(st[x]);
+            }
+            float hi = k.s, lo = -k.c;
+            comp_reduce(hi, lo, G);
+            if (r < r1 && gl == 0) pr_store(P, r, add + d * (hi + lo), part, inv_n);
+        }
+        wave_lds_sync();
+    }
+}
+
 // Block-reduces the per-thread FP64 partials and stores them at slot `index`
 // (quantity q at partial[q * cap + index]). No fences and no counters: a
 // later kernel in the same stream reduces the slots.
@@ -340,6 +439,15 @@ __device__ __forceinline__ void pr_sweep_body(const PrSweep& P) {
     }
 
     write_block_partials(part, P.partial, P.partial_cap, P.partial_base + b);
+}
+
+// The HOT-layout sweep (its own kernel: its LDS footprint is the whole CU's).
+__device__ __forceinline__ void pr_hot_sweep_body(const PrSweep& P) {
+    const double addd =
+        (1.0 - (double)P.damping) / (double)P.n + (double)P.damping * P.acc_in->dangling / (double)P.n;
+    PrPart part;
+    pr_hot_body(P, (float)addd, P.damping, 1.0f / (float)P.n, part);
+    write_block_partials(part, P.partial, P.partial_cap, P.partial_base + blockIdx.x);
 }
 
 // Multi-chunk long rows: combine the chunk partials of each row in chunk order

@@ -144,6 +144,11 @@ typedef struct ggpu_pr_opts {
  * in-degree (small_max, large_min, medium_lanes). */
 #define GGPU_PR_LAYOUT_STREAM 0
 #define GGPU_PR_LAYOUT_BINS 1
+/* HOT: one 1024-lane workgroup per compute unit for the whole sweep, the
+ * contributions of vertices [0, 30720) held in LDS (with ORDER_DEGREE those
+ * are the most gathered ones), wave-sized row blocks of <= 512 entries.
+ * Jacobi sweeps only: in-place sweeps with this layout use STREAM. */
+#define GGPU_PR_LAYOUT_HOT 2
 void ggpu_pr_opts_default(ggpu_pr_opts* o);
 /* "gpu_pull_scalar" | "gpu_pull_vector" | "gpu_pull_adaptive" | "gpu_pull_gs" -> GGPU_PR_* */
 int ggpu_pr_parse_variant(const char* name, int* variant);

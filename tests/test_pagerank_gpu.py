@@ -30,6 +30,8 @@ CFGS = [("gpu_pull_scalar", {}), ("gpu_pull_vector", {}),
         ("gpu_pull_adaptive", dict(order=G.PR_ORDER_DEGREE)),               # stream on the degree-ordered copy
         ("gpu_pull_adaptive", dict(order=G.PR_ORDER_DEGREE, layout=1)),
         ("gpu_pull_scalar", dict(order=G.PR_ORDER_DEGREE)),
+        ("gpu_pull_adaptive", dict(layout=G.PR_LAYOUT_HOT)),                # LDS-cached contributions
+        ("gpu_pull_adaptive", dict(layout=G.PR_LAYOUT_HOT, order=G.PR_ORDER_DEGREE)),
         ("gpu_pull_adaptive", dict(layout=B)),                              # the reference's 16 / 256 bins
         ("gpu_pull_adaptive", dict(layout=B, small_max=1 << 30, large_min=1 << 31)),  # all lane-per-vertex
         ("gpu_pull_adaptive", dict(layout=B, small_max=0, large_min=0)),              # all workgroup-per-vertex
@@ -129,7 +131,8 @@ def test_inplace_jacobi_variants_converge(name):
     conv, _, _ = O.pagerank(t.row_ptr, t.col_idx, od, 0.85, 1000, 1e-10)
     pr = G.GpuPageRank(t, od)
     for v, kw in (("gpu_pull_scalar", {}), ("gpu_pull_vector", {}), ("gpu_pull_adaptive", {}),
-                  ("gpu_pull_adaptive", dict(layout=G.PR_LAYOUT_BINS))):
+                  ("gpu_pull_adaptive", dict(layout=G.PR_LAYOUT_BINS)),
+                  ("gpu_pull_adaptive", dict(layout=G.PR_LAYOUT_HOT))):  # in place: row blocks
         pr.run(v, G.pr_options(iters=200, in_place=1, **kw))
         c = O.check_ranks(conv, pr.ranks())
         assert O.ranks_ok(*c), (v, kw, c)
@@ -144,6 +147,8 @@ def test_pr_errors_are_reported():
         pr.run("gpu_pull_adaptive", G.pr_options(tg=100))
     with pytest.raises(ValueError):
         pr.run("gpu_pull_adaptive", G.pr_options(medium_lanes=3))
+    with pytest.raises(ValueError):
+        pr.run("gpu_pull_adaptive", G.pr_options(layout=3))
 
 
 @pytest.mark.parametrize("name", ["rmat12_directed", "grid100x100", "two_components", "path1000"])
@@ -185,8 +190,11 @@ def test_stream_layout_long_rows_and_autotune():
         for tg in (64, 256, 1024):
             pr.run("gpu_pull_adaptive", G.pr_options(iters=20, tg=tg))
             assert O.ranks_ok(*O.check_ranks(ref, pr.ranks())), (name, tg)
+        for order in (G.PR_ORDER_GIVEN, G.PR_ORDER_DEGREE):  # hub: 196 wave chunks
+            pr.run("gpu_pull_adaptive", G.pr_options(iters=20, layout=G.PR_LAYOUT_HOT, order=order))
+            assert O.ranks_ok(*O.check_ranks(ref, pr.ranks())), (name, "hot", order)
         o = pr.autotune(iters=3)
-        pr.run("gpu_pull_adaptive", G.pr_options(iters=20, layout=o.layout, small_max=o.small_max,
+        pr.run("gpu_pull_adaptive", G.pr_options(iters=20, layout=o.layout, order=o.order, small_max=o.small_max,
                                                  large_min=o.large_min, medium_lanes=o.medium_lanes))
         assert O.ranks_ok(*O.check_ranks(ref, pr.ranks())), (name, o.layout, o.small_max, o.large_min)
     # a hub with 10000 in-neighbours: 2 chunks of the long-row path
@@ -195,5 +203,23 @@ def test_stream_layout_long_rows_and_autotune():
     t = g.transpose()
     ref, _, _ = O.pagerank(t.row_ptr, t.col_idx, g.degrees(), 0.85, 50, 0.0)
     pr = G.GpuPageRank(t, g.degrees())
-    pr.run("gpu_pull_adaptive", G.pr_options(iters=50))
-    assert O.ranks_ok(*O.check_ranks(ref, pr.ranks()))
+    for layout in (G.PR_LAYOUT_STREAM, G.PR_LAYOUT_HOT):
+        pr.run("gpu_pull_adaptive", G.pr_options(iters=50, layout=layout))
+        assert O.ranks_ok(*O.check_ranks(ref, pr.ranks())), layout
+
+
+def test_hot_layout_cache_boundary():
+    """More vertices than the LDS cache holds (65536 > 30720): gathers below
+    the boundary come from LDS, the rest from HBM; both orders, fixed sweeps
+    and tol mode, against the FP64 oracle with the acceptance rule above."""
+    g = G.Graph.named("rmat16_c1")
+    od = g.degrees()
+    pr = G.GpuPageRank(g, od)
+    for tol, iters in ((0.0, 20), (1e-6, 1000)):
+        ref, ref_iters, _ = O.pagerank(g.row_ptr, g.col_idx, od, 0.85, iters, tol)
+        for order in (G.PR_ORDER_GIVEN, G.PR_ORDER_DEGREE):
+            res = pr.run("gpu_pull_adaptive", G.pr_options(iters=iters, tol=tol, layout=G.PR_LAYOUT_HOT, order=order))
+            l1, rel, s = O.check_ranks(ref, pr.ranks())
+            assert l1 < 1e-4 and rel < 1e-3 and abs(s - 1) < 1e-4, (tol, order, l1, rel, s)
+            if tol > 0 and not res.floor_stop:
+                assert abs(res.iters - ref_iters) <= 3
