@@ -43,6 +43,7 @@ struct DlState {
     LoopState st;
     PolicyView pv;
     uint32_t started, prev_action, levels, done;
+    uint32_t sb_more, sb_pad;  // persistent: the single-workgroup episode ended with the traversal unfinished
     uint64_t t_prev;
     double tick_ms;
 };
@@ -59,6 +60,7 @@ struct DlDev {
     ggpu_level* trace;              // null: no trace
     uint32_t* bar;                  // persistent barrier words, then abort
     uint32_t* abort;
+    uint32_t* sb_flag;              // persistent: single-workgroup episodes finished (monotonic)
     uint32_t n_heavy, n, nb, trace_cap;
     uint64_t m;
 };
@@ -249,48 +251,241 @@ __device__ LevelOut dl_totals(const unsigned long long* part_slot, uint32_t bloc
     return r;
 }
 
-// The decision step shared by both loops. Every thread of the calling block
-// keeps the same LoopState; wave 0 evaluates the learned policy. Returns
-// false when the traversal is over. `writer`: this block writes the trace.
-__device__ bool dl_decide(const DlDev& D, LoopState& st, const PolicyView& pv, uint32_t& prev_action, bool started,
-                          const unsigned long long* part_slot, uint32_t blocks, bool writer, uint64_t& t_prev,
-                          double tick_ms, const float* mlp_lds) {
-    __shared__ float s_y[3];
-    if (started) {
-        const LevelOut r = dl_totals(part_slot, blocks);
-        if (writer && threadIdx.x == 0 && D.trace && st.level < D.trace_cap) {
-            const uint64_t now = wall_clock64();
-            ggpu_level& L = D.trace[st.level];
-            if (prev_action == GGPU_ACTION_BU) L.edges = r.examined;
-            L.time_ms = (double)(now - t_prev) * tick_ms;
-        }
-        fold_level(st, r, pv, D.n);
+// The loop state of one workgroup, in LDS: one copy per workgroup instead of
+// one per lane (it is wave-uniform; in registers it was spilled every level).
+struct DlBlock {
+    LoopState st;
+    PolicyView pv;
+    uint64_t t_prev;   // writer: wall clock when the current level was decided
+    uint32_t prev_action;
+    float y[3];        // learned policy outputs
+};
+
+// The learned policy's outputs for B.st into B.y; wave 0. Out of line: the
+// network's registers would otherwise be reserved across every level's code.
+__device__ __noinline__ void dl_policy_eval(DlBlock& B, const float* mlp_lds) {
+    float f[5], y[3];
+    policy_features(B.st.size, B.st.total, B.st.unvisited_edges, B.st.level, B.pv.mean_degree, f);
+    mlp_forward_wave(mlp_lds, f, y);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) B.y[q] = y[q];
+}
+
+// Thread 0's share of a decision.
+__device__ void dl_fold_one(const DlDev& D, DlBlock& B, const LevelOut& r, bool writer, double tick_ms) {
+    if (writer && D.trace && B.st.level < D.trace_cap) {
+        const uint64_t now = wall_clock64();
+        ggpu_level& L = D.trace[B.st.level];
+        if (B.prev_action == GGPU_ACTION_BU) L.edges = r.examined;
+        L.time_ms = (double)(now - B.t_prev) * tick_ms;
     }
-    if (st.size == 0) return false;
-    const bool learned = pv.mode == GGPU_POLICY_LEARNED && pv.mlp && pv.variant != GGPU_BFS_EDGE &&
-                         pv.variant != GGPU_BFS_BINNED;
-    if (learned) {
-        if (threadIdx.x < kWave) {
-            float f[5], y[3];
-            policy_features(st.size, st.total, st.unvisited_edges, st.level, pv.mean_degree, f);
-            mlp_forward_wave(mlp_lds, f, y);
-            if (threadIdx.x == 0)
-                for (int q = 0; q < 3; ++q) s_y[q] = y[q];
+    fold_level(B.st, r, B.pv, D.n);
+}
+
+__device__ void dl_decide_one(const DlDev& D, DlBlock& B, bool learned, bool writer) {
+    decide_action(B.st, B.pv, D.n, learned ? B.y : nullptr);
+    B.prev_action = B.st.action;
+    if (writer) {
+        B.t_prev = wall_clock64();
+        if (D.trace && B.st.level < D.trace_cap) {
+            LevelOut none;
+            none.examined = B.st.total;
+            D.trace[B.st.level] = trace_record(B.st, none, 'G', 0.0);
         }
+    }
+}
+
+// The decision step shared by both loops, called by every thread of the
+// block: thread 0 folds the finished level's totals `r` (null before the
+// first level) into B.st and decides the next action; wave 0 evaluates the
+// learned policy. Returns false when the traversal is over. `writer`: this
+// block writes the trace.
+__device__ bool dl_decide_core(const DlDev& D, DlBlock& B, const LevelOut* r, bool writer, double tick_ms,
+                               const float* mlp_lds) {
+    if (r) {
+        if (threadIdx.x == 0) dl_fold_one(D, B, *r, writer, tick_ms);
         __syncthreads();
     }
-    decide_action(st, pv, D.n, learned ? s_y : nullptr);
-    prev_action = st.action;
-    if (writer && threadIdx.x == 0) {
-        t_prev = wall_clock64();
-        if (D.trace && st.level < D.trace_cap) {
-            LevelOut none;
-            none.examined = st.total;
-            D.trace[st.level] = trace_record(st, none, 'G', 0.0);
+    if (B.st.size == 0) return false;
+    const bool learned = B.pv.mode == GGPU_POLICY_LEARNED && B.pv.mlp && B.pv.variant != GGPU_BFS_EDGE &&
+                         B.pv.variant != GGPU_BFS_BINNED;
+    if (learned && threadIdx.x < kWave) dl_policy_eval(B, mlp_lds);
+    if (learned) __syncthreads();
+    if (threadIdx.x == 0) dl_decide_one(D, B, learned, writer);
+    __syncthreads();
+    return true;
+}
+
+__device__ bool dl_decide(const DlDev& D, DlBlock& B, bool started, const unsigned long long* part_slot,
+                          uint32_t blocks, bool writer, double tick_ms, const float* mlp_lds) {
+    LevelOut r;
+    if (started) r = dl_totals(part_slot, blocks);
+    return dl_decide_core(D, B, started ? &r : nullptr, writer, tick_ms, mlp_lds);
+}
+
+// ------------------------------------------------- single-workgroup levels
+// A level whose frontier is small costs the full-grid loop a chain of
+// cross-XCD round trips (coherent counter reads, device-scope atomics, the
+// barrier) that does not shrink with the frontier. While the decision is top-
+// down and the frontier has at most kSbMax vertices and kSbEdges edges, the
+// persistent kernel runs levels on workgroup 0 alone: frontier queues in LDS,
+// visited and frontier bits updated with workgroup-scope atomics in that
+// XCD's L2, no grid barrier; the other workgroups wait on a flag. A level is
+// edge-parallel: the frontier's row offsets are loaded once (their sum is
+// also the previous level's next_edges) and scanned into an edge -> vertex
+// map in LDS, then every edge's col_idx load and visited atomic is issued at
+// once, so a level is about three memory round trips. The decisions come from the same fold / decide code with the same
+// counters (found, next_edges), so the trace equals the other loops'; the
+// frontier bitmaps are kept exactly as the grid loop keeps them, so either
+// loop can take over at any level.
+constexpr uint32_t kSbMax = 4096;
+constexpr uint32_t kSbEdges = 1u << 14;
+constexpr uint32_t kSbBatch = 8;  // edges per lane in flight (kSbEdges / blockDim rounds at most 2)
+
+struct SbShared {
+    uint32_t q[3][kSbMax];    // frontier queues: current, next, previous (whose bits are cleared)
+    uint32_t off[kSbMax];     // exclusive prefix of the current frontier's degrees
+    uint32_t vb[kSbMax];      // row start of each current frontier vertex
+    uint16_t map[kSbEdges];   // frontier index of each edge of the level
+    uint32_t qn[3];           // queue counts (may exceed kSbMax: then the queue is cut)
+    uint64_t wsum[2][kDlThreads / kWave];
+};
+
+__device__ __forceinline__ bool sb_ok(const LoopState& st) {
+    return st.action != GGPU_ACTION_BU && st.size <= kSbMax && st.total <= kSbEdges;
+}
+
+// Exclusive prefix over the block (1024 threads) of a u64 per thread, and
+// the block total of a second u64; returns the first's total.
+__device__ __forceinline__ uint64_t sb_scan(uint64_t v, uint64_t& excl, uint64_t v2, uint64_t& total2,
+                                            uint64_t (*wsum)[kDlThreads / kWave]) {
+    const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
+    uint64_t x = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint64_t y = __shfl_up(x, d, kWave);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    const uint64_t x2 = wave_sum(v2);
+    __syncthreads();
+    if (lane == kWave - 1) wsum[0][w] = x;
+    if (lane == 0) wsum[1][w] = x2;
+    __syncthreads();
+    uint64_t before = 0, total = 0, t2 = 0;
+    for (uint32_t i = 0; i < kDlThreads / kWave; ++i) {
+        before += i < w ? wsum[0][i] : 0ull;
+        total += wsum[0][i];
+        t2 += wsum[1][i];
+    }
+    excl = before + x - v;
+    total2 = t2;
+    return total;
+}
+
+// Workgroup 0 only. On entry B.st is decided for level L = B.st.level with
+// sb_ok(B.st); fb slot L % 3 holds the frontier, slot (L + 1) % 3 is clear,
+// slot (L + 2) % 3 holds the previous frontier. Runs levels until the
+// traversal ends (returns false) or a level is decided that sb_ok rejects
+// (returns true with B.st decided for that level and the slots as on entry).
+__device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const float* mlp_lds, SbShared& S) {
+    const uint32_t tid = threadIdx.x;
+    if (tid < 3) S.qn[tid] = 0;
+    __syncthreads();
+    {  // the frontier from its bitmap; the previous frontier's slot cleared in full
+        const uint64_t* fb = D.fb + (size_t)(B.st.level % 3) * D.nb;
+        uint64_t* fold = D.fb + (size_t)((B.st.level + 2) % 3) * D.nb;
+        for (uint32_t i = tid; i < D.nb; i += blockDim.x) {
+            uint64_t w = fb[i];
+            fold[i] = 0ull;
+            while (w) {
+                const uint32_t b = (uint32_t)__ffsll((long long)w) - 1;
+                w &= w - 1;
+                const uint32_t p = atomicAdd(&S.qn[0], 1u);
+                if (p < kSbMax) S.q[0][p] = i * 64 + b;
+            }
         }
     }
-    if (learned) __syncthreads();  // s_y is rewritten at the next level
-    return true;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    uint64_t cut_edges = 0;  // degrees of vertices found beyond the queue's capacity (this thread's)
+    for (uint32_t cur = 0, first = 1;; first = 0) {
+        const uint32_t nxt = (cur + 1) % 3, prv = (cur + 2) % 3;
+        const uint32_t found = S.qn[cur], qn = min(found, kSbMax), pn = min(S.qn[prv], kSbMax);
+        // the frontier's degrees: four consecutive vertices per thread
+        uint32_t b4[4], d4[4];
+        uint64_t dsum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t i = tid * 4 + k;
+            b4[k] = d4[k] = 0;
+            if (i < qn) {
+                const uint32_t v = S.q[cur][i];
+                b4[k] = D.rp[v];
+                d4[k] = D.rp[v + 1] - b4[k];
+            }
+            dsum += d4[k];
+        }
+        uint64_t excl, cut;
+        const uint64_t edges = sb_scan(dsum, excl, cut_edges, cut, S.wsum);
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {  // offsets and the edge map (used only while edges <= kSbEdges)
+            const uint32_t i = tid * 4 + k;
+            if (i < qn && edges <= kSbEdges) {
+                S.off[i] = (uint32_t)excl;
+                S.vb[i] = b4[k];
+                for (uint32_t x = 0; x < d4[k]; ++x) S.map[excl + x] = (uint16_t)i;
+            }
+            excl += d4[k];
+        }
+        if (!first) {
+            LevelOut r;
+            r.found = found;
+            r.next_edges = edges + cut;
+            if (!dl_decide_core(D, B, &r, true, tick_ms, mlp_lds)) return false;
+            if (found > kSbMax || !sb_ok(B.st)) return true;
+        }
+        cut_edges = 0;
+        const uint32_t level = B.st.level, depth = level + 1;
+        uint64_t* fbn = D.fb + (size_t)((level + 1) % 3) * D.nb;
+        uint64_t* fold = D.fb + (size_t)((level + 2) % 3) * D.nb;
+        for (uint32_t i = tid; i < pn; i += blockDim.x) fold[S.q[prv][i] >> 6] = 0ull;
+        if (tid == 0) S.qn[nxt] = 0;
+        __syncthreads();
+        // edge-parallel expansion: edge e0 + k * blockDim (consecutive edges in a wave), kSbBatch in flight
+        const uint32_t E = (uint32_t)edges;
+        for (uint32_t e0 = tid; e0 < E; e0 += blockDim.x * kSbBatch) {
+            uint32_t u[kSbBatch];
+#pragma unroll
+            for (uint32_t k = 0; k < kSbBatch; ++k) {
+                const uint32_t e = e0 + k * blockDim.x;
+                u[k] = 0;
+                if (e < E) {
+                    const uint32_t i = S.map[e];
+                    u[k] = D.ci[S.vb[i] + (e - S.off[i])];
+                }
+            }
+            uint64_t old[kSbBatch];
+#pragma unroll
+            for (uint32_t k = 0; k < kSbBatch; ++k)
+                old[k] = e0 + k * blockDim.x < E
+                             ? __hip_atomic_fetch_or(&D.vis[u[k] >> 6], 1ull << (u[k] & 63), __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP)
+                             : ~0ull;
+#pragma unroll
+            for (uint32_t k = 0; k < kSbBatch; ++k) {
+                const uint64_t bit = 1ull << (u[k] & 63);
+                if (old[k] & bit) continue;
+                D.dist[u[k]] = depth;
+                __hip_atomic_fetch_or(&fbn[u[k] >> 6], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const uint32_t p = atomicAdd(&S.qn[nxt], 1u);
+                if (p < kSbMax) S.q[nxt][p] = u[k];
+                else cut_edges += D.rp[u[k] + 1] - D.rp[u[k]];
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);  // this level's stores and atomics are in L2 before the next reads
+        __syncthreads();
+        cur = nxt;
+    }
 }
 
 // --------------------------------------------------------------- kernels
@@ -311,29 +506,81 @@ __global__ void k_dl_seed(DlDev D, uint32_t src) { D.dist[src] = 0; }
 
 __global__ void __launch_bounds__(kDlThreads) k_dl_persistent(DlDev D) {
     __shared__ float s_mlp[GGPU_MLP_PARAMS];
-    LoopState st = D.ds->st;
-    const PolicyView pv = D.ds->pv;
+    __shared__ DlBlock s_b;
+    __shared__ SbShared s_sb;
+    __shared__ uint32_t s_more;
     const double tick_ms = D.ds->tick_ms;
-    if (pv.mlp)
-        for (uint32_t i = threadIdx.x; i < GGPU_MLP_PARAMS; i += blockDim.x) s_mlp[i] = pv.mlp[i];
+    if (threadIdx.x == 0) {
+        s_b.st = D.ds->st;
+        s_b.pv = D.ds->pv;
+        s_b.t_prev = 0;
+        s_b.prev_action = 0;
+    }
     __syncthreads();
-    uint32_t prev_action = 0;
-    uint64_t t_prev = 0;
+    if (s_b.pv.mlp)
+        for (uint32_t i = threadIdx.x; i < GGPU_MLP_PARAMS; i += blockDim.x) s_mlp[i] = s_b.pv.mlp[i];
+    __syncthreads();
     const bool writer = blockIdx.x == 0;
     bool started = false;
+    uint32_t episodes = 0;
     for (;;) {
-        // the counters of level L (still st.level before the fold) are in slot (L + 1) & 1
-        if (!dl_decide(D, st, pv, prev_action, started,
-                       D.part + (size_t)((st.level + 1) & 1) * gridDim.x * kDlParts, gridDim.x, writer, t_prev,
-                       tick_ms, s_mlp))
+        // the counters of level L (still s_b.st.level before the fold) are in slot (L + 1) & 1
+        if (!dl_decide(D, s_b, started, D.part + (size_t)((s_b.st.level + 1) & 1) * gridDim.x * kDlParts,
+                       gridDim.x, writer, tick_ms, s_mlp))
             break;
         started = true;
-        dl_level(D, st.level, st.action, D.part + (size_t)((st.level + 1) & 1) * gridDim.x * kDlParts);
+        if (sb_ok(s_b.st)) {  // the same decision in every workgroup
+            ++episodes;
+            if (blockIdx.x == 0) {
+                // acquire: visited / frontier words other XCDs changed at the coherence point
+                if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                __syncthreads();
+                const bool more = sb_levels(D, s_b, tick_ms, s_mlp, s_sb);
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    D.ds->st = s_b.st;
+                    D.ds->prev_action = s_b.prev_action;
+                    D.ds->sb_more = more ? 1u : 0u;
+                    // release: this XCD's L2 (visited, frontier, dist, state) to the coherence point
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                    __hip_atomic_fetch_add(D.sb_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (!more) break;
+            } else {
+                if (threadIdx.x == 0) {
+                    uint64_t spins = 0;
+                    s_more = 1;
+                    while (load_coherent(D.sb_flag) < episodes) {
+                        __builtin_amdgcn_s_sleep(8);
+                        if (++spins > kBarSpinLimit || load_coherent(D.abort)) {
+                            __hip_atomic_store(D.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            s_more = 2;
+                            break;
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    if (s_more == 1) {
+                        s_more = D.ds->sb_more;
+                        s_b.st = D.ds->st;
+                        s_b.prev_action = D.ds->prev_action;
+                    }
+                }
+                __syncthreads();
+                const uint32_t more = s_more;
+                __syncthreads();  // s_more is rewritten at the next episode
+                if (more == 2) return;
+                if (!more) break;
+            }
+        }
+        // level L on the whole grid (decided above, or by workgroup 0 in its last level)
+        const uint32_t level = s_b.st.level, action = s_b.st.action;
+        dl_level(D, level, action, D.part + (size_t)((level + 1) & 1) * gridDim.x * kDlParts);
         if (!grid_barrier<kAcquireL1>(D.bar, D.abort)) return;
     }
     if (writer && threadIdx.x == 0) {
-        D.ds->st = st;
-        D.ds->levels = st.level;
+        D.ds->st = s_b.st;
+        D.ds->levels = s_b.st.level;
         D.ds->done = 1;
     }
 }
@@ -341,29 +588,30 @@ __global__ void __launch_bounds__(kDlThreads) k_dl_persistent(DlDev D) {
 // Graph mode: one block of one wave advances the state...
 __global__ void __launch_bounds__(kWave) k_dl_decide(DlDev D, uint32_t level_blocks) {
     __shared__ float s_mlp[GGPU_MLP_PARAMS];
-    __shared__ LoopState s_st;
+    __shared__ DlBlock s_b;
     DlState* ds = D.ds;
     if (ds->done) return;
-    const PolicyView pv = ds->pv;
-    if (pv.mlp)
-        for (uint32_t i = threadIdx.x; i < GGPU_MLP_PARAMS; i += blockDim.x) s_mlp[i] = pv.mlp[i];
-    if (threadIdx.x == 0) s_st = ds->st;
-    __syncthreads();
-    LoopState st = s_st;
-    uint32_t prev_action = ds->prev_action;
-    uint64_t t_prev = ds->t_prev;
-    const bool started = ds->started != 0;
-    const bool more = dl_decide(D, st, pv, prev_action, started,
-                                D.part + (size_t)((st.level + 1) & 1) * level_blocks * kDlParts, level_blocks, true,
-                                t_prev, ds->tick_ms, s_mlp);
     if (threadIdx.x == 0) {
-        ds->st = st;
-        ds->prev_action = prev_action;
-        ds->t_prev = t_prev;
+        s_b.st = ds->st;
+        s_b.pv = ds->pv;
+        s_b.prev_action = ds->prev_action;
+        s_b.t_prev = ds->t_prev;
+    }
+    __syncthreads();
+    if (s_b.pv.mlp)
+        for (uint32_t i = threadIdx.x; i < GGPU_MLP_PARAMS; i += blockDim.x) s_mlp[i] = s_b.pv.mlp[i];
+    __syncthreads();
+    const bool started = ds->started != 0;
+    const bool more = dl_decide(D, s_b, started, D.part + (size_t)((s_b.st.level + 1) & 1) * level_blocks * kDlParts,
+                                level_blocks, true, ds->tick_ms, s_mlp);
+    if (threadIdx.x == 0) {
+        ds->st = s_b.st;
+        ds->prev_action = s_b.prev_action;
+        ds->t_prev = s_b.t_prev;
         ds->started = 1;
         if (!more) {
             ds->done = 1;
-            ds->levels = st.level;
+            ds->levels = s_b.st.level;
         }
     }
 }
@@ -461,6 +709,7 @@ void BfsEngine::device_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, s
     D.trace = trace ? R.trace.p : nullptr;
     D.bar = R.ctl.p;
     D.abort = R.ctl.p + kBarWords;
+    D.sb_flag = R.ctl.p + kBarWords + 1;
     D.n_heavy = n_heavy;
     D.n = n;
     D.nb = nb;
