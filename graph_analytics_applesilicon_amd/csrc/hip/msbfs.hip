@@ -303,31 +303,39 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, MsAcc& a) {
 // not written, so the filter is exact for earlier levels and duplicates of
 // this level merge in next). Collect (after a barrier): each marked vertex
 // takes its new bits, clears next, and fb_{l+1} is rewritten exactly.
-template <int W>
+// kMark: also mark u in the next frontier bitmap fbn (the collect then visits
+// marked words only); without it the collect scans every vertex. Bits already
+// in next (a plain read; a stale one only costs the atomic) are not pushed
+// again: on a skewed graph most pushes of a big level land on a few hubs.
+template <int W, bool kMark>
 __device__ __forceinline__ void ms_push4(const MsDev& D, uint64_t* fbn, const uint32_t (&u)[4], uint32_t c,
                                          const uint64_t (&f)[W]) {
-    uint64_t vis[4][W];
+    uint64_t vis[4][W], nxt[4][W];
 #pragma unroll
     for (uint32_t i = 0; i < 4; ++i)
 #pragma unroll
-        for (int w = 0; w < W; ++w) vis[i][w] = (i < c && f[w]) ? D.visit[(size_t)u[i] * W + w] : ~0ull;
+        for (int w = 0; w < W; ++w) {
+            const bool ok = i < c && f[w];
+            vis[i][w] = ok ? D.visit[(size_t)u[i] * W + w] : ~0ull;
+            nxt[i][w] = ok ? D.next[(size_t)u[i] * W + w] : ~0ull;
+        }
 #pragma unroll
     for (uint32_t i = 0; i < 4; ++i) {
         bool any = false;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            const uint64_t x = f[w] & ~vis[i][w];
+            const uint64_t x = f[w] & ~vis[i][w] & ~nxt[i][w];
             if (x) {
                 atomicOr((unsigned long long*)&D.next[(size_t)u[i] * W + w], (unsigned long long)x);
                 any = true;
             }
         }
-        if (any) atomicOr((unsigned long long*)&fbn[u[i] >> 6], (unsigned long long)(1ull << (u[i] & 63)));
+        if (kMark && any) atomicOr((unsigned long long*)&fbn[u[i] >> 6], (unsigned long long)(1ull << (u[i] & 63)));
     }
 }
 
 // Pushes edges [b, e) of a vertex with frontier bits f, `step` lanes apart.
-template <int W>
+template <int W, bool kMark>
 __device__ __forceinline__ void ms_push_range(const MsDev& D, uint64_t* fbn, uint32_t b, uint32_t e, uint32_t first,
                                               uint32_t step, const uint64_t (&f)[W]) {
     for (uint32_t x = b + first; x < e; x += 4 * step) {
@@ -339,11 +347,11 @@ __device__ __forceinline__ void ms_push_range(const MsDev& D, uint64_t* fbn, uin
             u[i] = y < e ? D.ci[y] : 0u;
             c += y < e;
         }
-        ms_push4<W>(D, fbn, u, c, f);
+        ms_push4<W, kMark>(D, fbn, u, c, f);
     }
 }
 
-template <int W>
+template <int W, bool kMark>
 __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex) {
     const uint32_t cs = level % D.C, ns = (level + 1) % D.C;
     const uint64_t* F = D.F + (size_t)cs * D.n * W;
@@ -370,7 +378,7 @@ __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex)
                         uint64_t f[W];
 #pragma unroll
                         for (int w = 0; w < W; ++w) f[w] = F[(size_t)v * W + w];
-                        ms_push_range<W>(D, fbn, b, e, 0, 1, f);
+                        ms_push_range<W, kMark>(D, fbn, b, e, 0, 1, f);
                     }
                 }
             } else {
@@ -388,7 +396,7 @@ __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex)
                     uint64_t f[W];
 #pragma unroll
                     for (int w = 0; w < W; ++w) f[w] = F[(size_t)vj * W + w];
-                    ms_push_range<W>(D, fbn, bj, ej, lane, kWave, f);
+                    ms_push_range<W, kMark>(D, fbn, bj, ej, lane, kWave, f);
                 }
             }
         }
@@ -411,7 +419,7 @@ __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex)
             uint64_t f[W];
 #pragma unroll
             for (int w = 0; w < W; ++w) f[w] = F[(size_t)v * W + w];
-            ms_push_range<W>(D, fbn, y, z, lane, kWave, f);
+            ms_push_range<W, kMark>(D, fbn, y, z, lane, kWave, f);
         }
     }
 }
@@ -455,6 +463,48 @@ __device__ void ms_td_collect(const MsDev& D, uint32_t level, MsAcc& a) {
         }
         if (marked) {
             store_through(&fbn[wi], found);
+            a.size += (uint64_t)__popcll(found);
+        }
+    }
+}
+
+// The collect of an unmarked push: every vertex's next words are read
+// (coalesced, 8 B per word and vertex); a wave owns 64 consecutive vertices,
+// so it writes their frontier word once instead of one atomic per push.
+template <int W>
+__device__ void ms_td_collect_all(const MsDev& D, uint32_t level, MsAcc& a) {
+    const uint32_t ns = (level + 1) % D.C;
+    uint64_t* Fn = D.F + (size_t)ns * D.n * W;
+    uint64_t* fbn = D.fb + (size_t)ns * D.nb;
+    const uint32_t lane = lane_id();
+    for (uint64_t b = (uint64_t)gwave() * kWave; b < D.n; b += (uint64_t)nwaves() * kWave) {
+        const uint32_t u = (uint32_t)b + lane;
+        uint32_t pc = 0;
+        if (u < D.n) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const size_t i = (size_t)u * W + w;
+                const uint64_t nx = D.next[i];
+                if (nx) {
+                    const uint64_t vis = D.visit[i];
+                    const uint64_t nw = nx & ~vis;
+                    store_through(&D.next[i], 0ull);
+                    if (nw) {
+                        store_through(&Fn[i], nw);
+                        store_through(&D.visit[i], vis | nw);
+                        pc += (uint32_t)__popcll(nw);
+                    }
+                }
+            }
+        }
+        const uint64_t found = __ballot(pc != 0);
+        if (pc) {
+            const uint64_t dg = D.rp[u + 1] - D.rp[u];
+            a.pairs += dg * pc;
+            a.edges += dg;
+        }
+        if (lane == 0 && found) {
+            store_through(&fbn[b >> 6], found);
             a.size += (uint64_t)__popcll(found);
         }
     }
@@ -642,9 +692,14 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
         if (act == GGPU_ACTION_BU) {
             ms_bottom_up<W>(D, level, a);
         } else {
-            ms_td_push<W>(D, level, act == GGPU_ACTION_TD_BINNED);
+            // a big push marks nothing and the collect scans every vertex (fewer
+            // memory operations than one contended bitmap atomic per push)
+            const bool scan_all = (uint64_t)t.edges * 8 > (uint64_t)D.n;
+            if (scan_all) ms_td_push<W, false>(D, level, act == GGPU_ACTION_TD_BINNED);
+            else ms_td_push<W, true>(D, level, act == GGPU_ACTION_TD_BINNED);
             if (!grid_barrier(D.bar, D.abort)) return;
-            ms_td_collect<W>(D, level, a);
+            if (scan_all) ms_td_collect_all<W>(D, level, a);
+            else ms_td_collect<W>(D, level, a);
         }
         ms_flush(D, (level + 1) & 1, a);
         if (!grid_barrier(D.bar, D.abort)) return;
