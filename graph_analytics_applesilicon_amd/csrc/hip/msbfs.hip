@@ -40,10 +40,17 @@ constexpr uint32_t kMsChunk = 1024;     // edges per item
 constexpr uint32_t kMsSlots = 16;       // level slots (<= 64: six depth planes)
 constexpr uint32_t kMsTraceCap = 1u << 16;
 constexpr uint32_t kPartWords = 4;
+// Bottom-up LDS cache: the frontier words of the highest-degree vertices
+// (kMsHotWords / W of them; 16384 at W = 1 take 41% of rmat22's arcs). In
+// the bottom-up copy of col_idx an arc to such a vertex holds kMsHotBit | slot.
+constexpr uint32_t kMsHotWords = 16384;  // 128 KiB of LDS
+constexpr uint32_t kMsHotBit = 1u << 31;
 
 struct MsDev {
     const uint32_t* rp;
     const uint32_t* ci;
+    const uint32_t* cib;           // bottom-up col_idx: kMsHotBit | slot for arcs to hot vertices
+    const uint32_t* hot;           // n_hot hot vertex ids, by slot
     uint64_t* visit;               // n * W
     uint64_t* next;                // n * W, top-down push target (zero between levels)
     uint64_t* F;                   // C * n * W
@@ -64,6 +71,7 @@ struct MsDev {
     uint64_t m;
     uint64_t valid_last;           // valid bits of the last source word
     uint32_t n, k, nb, n_heavy, C, trace_cap, n_actions, policy;
+    uint32_t n_hot;                // <= kMsHotWords / W
 };
 
 struct MsAcc {
@@ -151,13 +159,22 @@ __device__ void ms_seed(const MsDev& D, MsAcc& a) {
 
 // ---------------------------------------------------------------- bottom-up
 template <int W>
-__device__ void ms_bottom_up(const MsDev& D, uint32_t level, MsAcc& a) {
+__device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, MsAcc& a) {
     const uint32_t cs = level % D.C, ns = (level + 1) % D.C;
     const uint64_t* F = D.F + (size_t)cs * D.n * W;
     const uint64_t* fb = D.fb + (size_t)cs * D.nb;
     uint64_t* Fn = D.F + (size_t)ns * D.n * W;
     uint64_t* fbn = D.fb + (size_t)ns * D.nb;
     const uint32_t lane = lane_id();
+    // this level's frontier words of the hot vertices (zero: not in the frontier);
+    // a level with little left to scan does not pay for the copy and reads col_idx
+    __shared__ uint64_t s_hot[kMsHotWords];
+    const uint32_t* cx = use_hot ? D.cib : D.ci;
+    if (use_hot) {
+        for (uint32_t i = threadIdx.x; i < D.n_hot * W; i += blockDim.x)
+            s_hot[i] = F[(size_t)D.hot[i / W] * W + i % W];
+        __syncthreads();
+    }
 
     // light vertices: lane per vertex, 64 consecutive vertices per wave
     for (uint32_t wd = gwave(); wd < D.nb; wd += nwaves()) {
@@ -181,15 +198,20 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, MsAcc& a) {
                         uint32_t u[4];
                         uint64_t f[4];
 #pragma unroll
-                        for (uint32_t i = 0; i < 4; ++i) u[i] = i < c ? D.ci[x + i] : 0u;
-#pragma unroll
-                        for (uint32_t i = 0; i < 4; ++i) f[i] = i < c ? fb[u[i] >> 6] >> (u[i] & 63) : 0ull;
+                        for (uint32_t i = 0; i < 4; ++i) u[i] = i < c ? cx[x + i] : 0u;
 #pragma unroll
                         for (uint32_t i = 0; i < 4; ++i)
-                            if (f[i] & 1ull) {
+                            f[i] = i < c && !(u[i] & kMsHotBit) ? fb[u[i] >> 6] >> (u[i] & 63) : 0ull;
+#pragma unroll
+                        for (uint32_t i = 0; i < 4; ++i) {
+                            if (i < c && (u[i] & kMsHotBit)) {
+#pragma unroll
+                                for (int w = 0; w < W; ++w) acc[w] |= s_hot[(u[i] & ~kMsHotBit) * W + w];
+                            } else if (f[i] & 1ull) {
 #pragma unroll
                                 for (int w = 0; w < W; ++w) acc[w] |= F[(size_t)u[i] * W + w];
                             }
+                        }
                         a.examined += c;
                         bool all = true;
 #pragma unroll
@@ -257,20 +279,26 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, MsAcc& a) {
             for (uint32_t xb = y; xb < z; xb += 4 * kWave) {
                 uint32_t u[4];
                 uint64_t f[4];
+                bool in[4];
 #pragma unroll
                 for (uint32_t i = 0; i < 4; ++i) {
                     const uint32_t x = xb + i * kWave + lane;
-                    u[i] = x < z ? D.ci[x] : 0xFFFFFFFFu;
+                    in[i] = x < z;
+                    u[i] = in[i] ? cx[x] : 0u;
                 }
 #pragma unroll
-                for (uint32_t i = 0; i < 4; ++i) f[i] = u[i] != 0xFFFFFFFFu ? fb[u[i] >> 6] >> (u[i] & 63) : 0ull;
+                for (uint32_t i = 0; i < 4; ++i)
+                    f[i] = in[i] && !(u[i] & kMsHotBit) ? fb[u[i] >> 6] >> (u[i] & 63) : 0ull;
 #pragma unroll
                 for (uint32_t i = 0; i < 4; ++i) {
-                    if (f[i] & 1ull) {
+                    if (in[i] && (u[i] & kMsHotBit)) {
+#pragma unroll
+                        for (int w = 0; w < W; ++w) acc[w] |= s_hot[(u[i] & ~kMsHotBit) * W + w];
+                    } else if (f[i] & 1ull) {
 #pragma unroll
                         for (int w = 0; w < W; ++w) acc[w] |= F[(size_t)u[i] * W + w];
                     }
-                    a.examined += u[i] != 0xFFFFFFFFu;
+                    a.examined += in[i];
                 }
                 bool all = true;
 #pragma unroll
@@ -690,7 +718,9 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
         }
         MsAcc a;
         if (act == GGPU_ACTION_BU) {
-            ms_bottom_up<W>(D, level, a);
+            // the hot copy (~25 us per level on 256 workgroups) pays while at least
+            // 1/32 of the pair-edges are unreached
+            ms_bottom_up<W>(D, level, pairs_u * 32 > total_pairs, a);
         } else {
             // a big push marks nothing and the collect scans every vertex (fewer
             // memory operations than one contended bitmap atomic per push)
@@ -723,6 +753,8 @@ struct MsState {
     DBuf<unsigned long long> part;
     DBuf<uint32_t> dist, sources, ctl;
     DBuf<uint4> heavy;
+    DBuf<uint32_t> cib, hot;  // bottom-up col_idx with hot slots, hot ids (for hot_w words per vertex)
+    uint32_t hot_w = 0, n_hot = 0;
     DBuf<ggpu_level> trace;
     DBuf<float> feat;
     DBuf<uint8_t> actions;
@@ -819,6 +851,30 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
                 GGPU_HIP_TRY(hipMemcpy(S.heavy.p, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice));
             S.heavy_ready = true;
         }
+        if (S.hot_w != W) {
+            // the highest-degree vertices (ties by id), kMsHotWords / W of them
+            const uint32_t H = n < (1u << 31) ? std::min<uint32_t>(kMsHotWords / W, n) : 0u;
+            std::vector<uint32_t> order(n);
+            for (uint32_t v = 0; v < n; ++v) order[v] = v;
+            std::partial_sort(order.begin(), order.begin() + H, order.end(), [&](uint32_t a, uint32_t b) {
+                const uint32_t da = e.host.deg(a), db = e.host.deg(b);
+                return da != db ? da > db : a < b;
+            });
+            std::vector<uint32_t> slot(n, 0xFFFFFFFFu);
+            for (uint32_t i = 0; i < H; ++i) slot[order[i]] = i;
+            std::vector<uint32_t> cib(e.host.ci.size());
+            for (size_t x = 0; x < cib.size(); ++x) {
+                const uint32_t u = e.host.ci[x];
+                cib[x] = slot[u] != 0xFFFFFFFFu ? (kMsHotBit | slot[u]) : u;
+            }
+            S.cib.alloc(cib.size() + 1);
+            if (!cib.empty())
+                GGPU_HIP_TRY(hipMemcpy(S.cib.p, cib.data(), cib.size() * 4, hipMemcpyHostToDevice));
+            S.hot.alloc(H + 1);
+            if (H) GGPU_HIP_TRY(hipMemcpy(S.hot.p, order.data(), (size_t)H * 4, hipMemcpyHostToDevice));
+            S.n_hot = H;
+            S.hot_w = W;
+        }
         // level slots: kMsSlots, fewer if device memory is short (at least 2)
         if (S.W != W) {
             S.visit.release();
@@ -866,6 +922,9 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         MsDev D{};
         D.rp = e.rp.p;
         D.ci = e.ci.p;
+        D.cib = S.cib.p;
+        D.hot = S.hot.p;
+        D.n_hot = S.n_hot;
         D.visit = S.visit.p;
         D.next = S.next.p;
         D.F = S.F.p;
