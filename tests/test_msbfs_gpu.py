@@ -95,12 +95,16 @@ def test_batched_all_same_source_and_state_reuse():
 
 
 def test_batched_larger_graph():
-    """A 65,536-vertex RMAT graph (rmat(16, 16, .57, .19, .19), permuted)."""
+    """A 65,536-vertex RMAT graph (rmat(16, 16, .57, .19, .19), permuted): more
+    vertices than the bottom-up LDS cache holds (16384 / W), so arcs to cached
+    and to uncached vertices mix; big top-down levels take the unmarked push
+    and the full collect, small ones the marked path; 64 and 256 sources."""
     g = G.Graph.named("rmat16_c1")
     bfs = G.GpuBfs(g)
-    src = np.array(g.pick_sources(64, 1), dtype=np.uint32)
-    for mode, kw in MODES.items():
-        _check(g, bfs, src, G.bfs_options(**kw), f"rmat16_c1 {mode}")
+    for k in (64, 256):
+        src = np.array(g.pick_sources(k, 1), dtype=np.uint32)
+        for mode, kw in MODES.items():
+            _check(g, bfs, src, G.bfs_options(**kw), f"rmat16_c1 k={k} {mode}")
 
 
 def test_batched_errors():
