@@ -295,6 +295,8 @@ __device__ void dl_decide_one(const DlDev& D, DlBlock& B, bool learned, bool wri
     }
 }
 
+__device__ bool dl_decide_rest(const DlDev& D, DlBlock& B, bool writer, const float* mlp_lds);
+
 // The decision step shared by both loops, called by every thread of the
 // block: thread 0 folds the finished level's totals `r` (null before the
 // first level) into B.st and decides the next action; wave 0 evaluates the
@@ -306,6 +308,11 @@ __device__ bool dl_decide_core(const DlDev& D, DlBlock& B, const LevelOut* r, bo
         if (threadIdx.x == 0) dl_fold_one(D, B, *r, writer, tick_ms);
         __syncthreads();
     }
+    return dl_decide_rest(D, B, writer, mlp_lds);
+}
+
+// The part of dl_decide_core after the fold (B.st folded and visible to the block).
+__device__ bool dl_decide_rest(const DlDev& D, DlBlock& B, bool writer, const float* mlp_lds) {
     if (B.st.size == 0) return false;
     const bool learned = B.pv.mode == GGPU_POLICY_LEARNED && B.pv.mlp && B.pv.variant != GGPU_BFS_EDGE &&
                          B.pv.variant != GGPU_BFS_BINNED;
@@ -388,7 +395,7 @@ __device__ __forceinline__ uint64_t sb_scan(uint64_t v, uint64_t& excl, uint64_t
 // traversal ends (returns false) or a level is decided that sb_ok rejects
 // (returns true with B.st decided for that level and the slots as on entry).
 __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const float* mlp_lds, SbShared& S) {
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = lane_id();
     if (tid < 3) S.qn[tid] = 0;
     __syncthreads();
     {  // the frontier from its bitmap; the previous frontier's slot cleared in full
@@ -427,6 +434,12 @@ __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const floa
         }
         uint64_t excl, cut;
         const uint64_t edges = sb_scan(dsum, excl, cut_edges, cut, S.wsum);
+        if (!first && tid == 0) {  // the previous level's fold, beside the edge map
+            LevelOut r;
+            r.found = found;
+            r.next_edges = edges + cut;
+            dl_fold_one(D, B, r, true, tick_ms);
+        }
 #pragma unroll
         for (uint32_t k = 0; k < 4; ++k) {  // offsets and the edge map (used only while edges <= kSbEdges)
             const uint32_t i = tid * 4 + k;
@@ -437,11 +450,9 @@ __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const floa
             }
             excl += d4[k];
         }
+        __syncthreads();
         if (!first) {
-            LevelOut r;
-            r.found = found;
-            r.next_edges = edges + cut;
-            if (!dl_decide_core(D, B, &r, true, tick_ms, mlp_lds)) return false;
+            if (!dl_decide_rest(D, B, true, mlp_lds)) return false;
             if (found > kSbMax || !sb_ok(B.st)) return true;
         }
         cut_edges = 0;
@@ -449,6 +460,9 @@ __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const floa
         uint64_t* fbn = D.fb + (size_t)((level + 1) % 3) * D.nb;
         uint64_t* fold = D.fb + (size_t)((level + 2) % 3) * D.nb;
         for (uint32_t i = tid; i < pn; i += blockDim.x) fold[S.q[prv][i] >> 6] = 0ull;
+        // every thread's clears (and its previous level's stores and atomics) land
+        // before any of this level's atomics on the slot just cleared
+        __builtin_amdgcn_s_waitcnt(0);
         if (tid == 0) S.qn[nxt] = 0;
         __syncthreads();
         // edge-parallel expansion: edge e0 + k * blockDim (consecutive edges in a wave), kSbBatch in flight
@@ -474,15 +488,25 @@ __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const floa
 #pragma unroll
             for (uint32_t k = 0; k < kSbBatch; ++k) {
                 const uint64_t bit = 1ull << (u[k] & 63);
-                if (old[k] & bit) continue;
-                D.dist[u[k]] = depth;
-                __hip_atomic_fetch_or(&fbn[u[k] >> 6], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                const uint32_t p = atomicAdd(&S.qn[nxt], 1u);
-                if (p < kSbMax) S.q[nxt][p] = u[k];
-                else cut_edges += D.rp[u[k] + 1] - D.rp[u[k]];
+                const bool fresh = !(old[k] & bit);
+                if (fresh) {
+                    D.dist[u[k]] = depth;
+                    __hip_atomic_fetch_or(&fbn[u[k] >> 6], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                // one queue reservation per wave
+                const uint64_t m = __ballot(fresh);
+                if (!m) continue;
+                uint32_t base = 0;
+                if (lane == (uint32_t)__ffsll((long long)m) - 1) base = atomicAdd(&S.qn[nxt], (uint32_t)__popcll(m));
+                base = __shfl(base, __ffsll((long long)m) - 1, kWave);
+                if (fresh) {
+                    const uint32_t p = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+                    if (p < kSbMax) S.q[nxt][p] = u[k];
+                    else cut_edges += D.rp[u[k] + 1] - D.rp[u[k]];
+                }
             }
         }
-        __builtin_amdgcn_s_waitcnt(0);  // this level's stores and atomics are in L2 before the next reads
+        // this level's stores and atomics are waited for after the next level's clears
         __syncthreads();
         cur = nxt;
     }
