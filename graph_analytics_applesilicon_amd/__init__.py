@@ -194,6 +194,7 @@ def _optional_hip_symbols():
         "ggpu_pr_autotune": ([vp, C.c_int, C.POINTER(PrOpts)], C.c_int),
         "ggpu_ppr_run": ([vp, _u32p, C.c_uint32, C.POINTER(PrOpts), C.POINTER(PrResult)], C.c_int),
         "ggpu_ppr_fetch": ([vp, fp], C.c_int),
+        "ggpu_ppr_autotune": ([vp, _u32p, C.c_uint32, C.c_int, C.POINTER(PrOpts)], C.c_int),
         "ggpu_policy_set": ([vp, C.c_int, fp, C.c_uint32], C.c_int),
         "ggpu_policy_clear": ([vp, C.c_int], C.c_int),
         "ggpu_policy_get": ([vp, C.c_int, fp, C.c_uint32, C.POINTER(C.c_int)], C.c_int),
@@ -605,6 +606,15 @@ class GpuPageRank:
         self.ctx.check(self.ctx.lib.ggpu_ppr_run(self._h, _ptr(s), len(s), C.byref(opts or pr_options()),
                                                  C.byref(res)))
         return res
+
+    def autotune_ppr(self, seeds, iters=3, opts: Optional[PrOpts] = None) -> PrOpts:
+        """Fastest (large_min, medium_lanes, tg) of the PPR block for these
+        seeds; the block then holds the last candidate's ranks."""
+        s = np.ascontiguousarray(seeds, dtype=np.uint32)
+        o = opts or pr_options()
+        self._ppr_b = len(s)
+        self.ctx.check(self.ctx.lib.ggpu_ppr_autotune(self._h, _ptr(s), len(s), iters, C.byref(o)))
+        return o
 
     def ppr_block(self) -> np.ndarray:
         b = self._ppr_b

@@ -631,7 +631,7 @@ struct ggpu_pr {
 namespace ggpu {
 PprView ppr_view(PrEngine& e) {
     const Space& g = e.sp[SP_GIVEN];
-    return PprView{e.ctx, e.n, g.rp.p, g.ci.p, g.odd.p, g.od.data()};
+    return PprView{e.ctx, e.n, g.rp.p, g.ci.p, g.odd.p, g.od.data(), g.in.rp.data()};
 }
 PprState& ppr_state(PrEngine& e) { return e.ppr; }
 }  // namespace ggpu
@@ -737,6 +737,14 @@ int ggpu_ppr_run(ggpu_pr* p, const uint32_t* seeds, uint32_t B, const ggpu_pr_op
         GGPU_HIP_TRY(hipSetDevice(p->ctx->device));
         ggpu_pr_result r = ppr_run(*p->eng, seeds, B, o);
         if (res) *res = r;
+    });
+}
+
+int ggpu_ppr_autotune(ggpu_pr* p, const uint32_t* seeds, uint32_t B, int iters, ggpu_pr_opts* opt) {
+    if (!p || !seeds || !opt) return GGPU_EINVAL;
+    return guarded(p->ctx, [&] {
+        GGPU_HIP_TRY(hipSetDevice(p->ctx->device));
+        ppr_autotune(*p->eng, seeds, B, iters, *opt);
     });
 }
 

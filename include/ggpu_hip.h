@@ -182,10 +182,17 @@ int ggpu_pr_fetch_ranks(ggpu_pr* p, float* out);
  * the fastest into opt (ranks are unaffected by the choice). */
 int ggpu_pr_autotune(ggpu_pr* p, int iters, ggpu_pr_opts* opt);
 
-/* Personalized PageRank block (no reference counterpart): B one-hot restart
- * vectors (seeds[b]) iterate together as an n x B FP32 block. Restart and
- * dangling mass of column b both return to seeds[b]. */
+/* Personalized PageRank block (no reference counterpart): B <= 256 one-hot
+ * restart vectors (seeds[b]) iterate together as an n x B FP32 block. Restart
+ * and dangling mass of column b both return to seeds[b]. Work assignment from
+ * opt: in-lists of >= large_min entries (>= 64) are cut into large_min-entry
+ * chunks, a wave each; shorter ones get medium_lanes lanes per vertex
+ * (clamped up to min(64, B rounded to a power of two)); tg workgroups. */
 int ggpu_ppr_run(ggpu_pr* p, const uint32_t* seeds, uint32_t B, const ggpu_pr_opts* opt, ggpu_pr_result* res);
+/* Autotuner of the block (no reference counterpart): times (large_min,
+ * medium_lanes, tg) candidates for `iters` sweeps of this seed block and
+ * writes the fastest into opt; the block holds the last candidate's ranks. */
+int ggpu_ppr_autotune(ggpu_pr* p, const uint32_t* seeds, uint32_t B, int iters, ggpu_pr_opts* opt);
 /* n*B floats, vertex-major: out[v*B + b]. */
 int ggpu_ppr_fetch(ggpu_pr* p, float* out);
 

@@ -151,6 +151,21 @@ def test_pr_errors_are_reported():
         pr.run("gpu_pull_adaptive", G.pr_options(layout=3))
 
 
+# PPR work assignments: the defaults (heavy in-lists >= 256 entries in chunks,
+# a wave per light vertex), every list of >= 64 entries chunked with lane
+# groups of 1 and 16, and no chunking at all.
+PPR_ASSIGN = [dict(), dict(large_min=64, medium_lanes=1, tg=64), dict(large_min=64, medium_lanes=16, tg=1024),
+              dict(large_min=0xFFFFFFFF, medium_lanes=64, tg=256)]
+
+
+def _ppr_check(tag, blk, refs, tol=2e-5):
+    for b, ref in enumerate(refs):
+        col = blk[:, b].astype(np.float64)
+        err = float(np.abs(col - ref).sum())
+        assert err < tol, f"{tag} col {b}: L1 {err:.3g}"
+        assert abs(float(col.sum()) - 1.0) < tol, f"{tag} col {b}: sum {col.sum()}"
+
+
 @pytest.mark.parametrize("name", ["rmat12_directed", "grid100x100", "two_components", "path1000"])
 def test_ppr_block_matches_oracle(name):
     g, t, od = _in_graph(name)
@@ -158,14 +173,47 @@ def test_ppr_block_matches_oracle(name):
     rng = np.random.default_rng(3)
     for B in (1, 5, 64, 100):
         seeds = rng.integers(0, g.n, size=B).astype(np.uint32)
-        pr.run_ppr(seeds, G.pr_options(iters=60))
-        blk = pr.ppr_block()
-        assert blk.shape == (g.n, B)
-        for b in range(B):
-            ref, _, _ = O.ppr(t.row_ptr, t.col_idx, od, int(seeds[b]), 0.85, 60)
-            err = float(np.abs(blk[:, b].astype(np.float64) - ref).sum())
-            assert err < 2e-5, f"{name} B={B} col {b} seed {seeds[b]}: L1 {err:.3g}"
-            assert abs(float(blk[:, b].astype(np.float64).sum()) - 1.0) < 2e-5
+        refs = [O.ppr(t.row_ptr, t.col_idx, od, int(s), 0.85, 60)[0] for s in seeds]
+        for kw in PPR_ASSIGN:
+            pr.run_ppr(seeds, G.pr_options(iters=60, **kw))
+            blk = pr.ppr_block()
+            assert blk.shape == (g.n, B)
+            _ppr_check(f"{name} B={B} {kw}", blk, refs)
+
+
+def test_ppr_hub_chunks():
+    """The star's hub has in-degree 100000: 98 chunks of 1024 entries (the last
+    one short) combined in chunk order, against a single wave walking the whole
+    list; 3 and 130 columns (one and three column passes), seeds on the hub
+    and on leaves."""
+    g, t, od = _in_graph("star100k")
+    pr = G.GpuPageRank(t, od)
+    hub = int(np.argmax(np.diff(t.row_ptr)))
+    for B in (3, 130):
+        seeds = np.array([hub] + [1 + 997 * i % (g.n - 1) for i in range(B - 1)], dtype=np.uint32)
+        refs = [O.ppr(t.row_ptr, t.col_idx, od, int(s), 0.85, 30)[0] for s in seeds]
+        for kw in (dict(large_min=1024), dict(large_min=1000, medium_lanes=4), dict(large_min=0xFFFFFFFF)):
+            pr.run_ppr(seeds, G.pr_options(iters=30, **kw))
+            _ppr_check(f"star B={B} {kw}", pr.ppr_block(), refs)
+
+
+def test_ppr_autotune_and_errors():
+    g, t, od = _in_graph("rmat12_directed")
+    pr = G.GpuPageRank(t, od)
+    seeds = np.arange(0, 8 * 97, 97, dtype=np.uint32)
+    o = pr.autotune_ppr(seeds, iters=2, opts=G.pr_options(iters=40))
+    assert o.large_min in (64, 128, 256, 1024) and o.medium_lanes in (8, 16, 32, 64) and o.tg in (256, 1024)
+    assert o.iters == 40
+    pr.run_ppr(seeds, o)
+    refs = [O.ppr(t.row_ptr, t.col_idx, od, int(s), 0.85, 40)[0] for s in seeds]
+    _ppr_check(f"autotuned {o.large_min} {o.medium_lanes} {o.tg}", pr.ppr_block(), refs)
+    for kw in (dict(large_min=63), dict(medium_lanes=3), dict(medium_lanes=128), dict(tg=96)):
+        with pytest.raises(ValueError):
+            pr.run_ppr(seeds, G.pr_options(**kw))
+    with pytest.raises(ValueError):
+        pr.run_ppr(np.zeros(257, dtype=np.uint32))
+    with pytest.raises(ValueError):
+        pr.run_ppr(np.array([g.n], dtype=np.uint32))
 
 
 def test_ppr_isolated_seed_is_one_hot():
