@@ -629,9 +629,12 @@ struct ggpu_pr {
 };
 
 namespace ggpu {
-PprView ppr_view(PrEngine& e) {
-    const Space& g = e.sp[SP_GIVEN];
-    return PprView{e.ctx, e.n, g.rp.p, g.ci.p, g.odd.p, g.od.data(), g.in.rp.data()};
+PprView ppr_view(PrEngine& e, uint32_t order) {
+    if (order > GGPU_PR_ORDER_DEGREE) throw ArgError("unknown vertex order");
+    const int id = order == GGPU_PR_ORDER_DEGREE ? SP_DEGREE : SP_GIVEN;
+    const Space& g = e.space(id);
+    return PprView{e.ctx, e.n, g.rp.p, g.ci.p, g.odd.p, g.od.data(), g.in.rp.data(),
+                   g.id_of.empty() ? nullptr : g.id_of.data(), id};
 }
 PprState& ppr_state(PrEngine& e) { return e.ppr; }
 }  // namespace ggpu

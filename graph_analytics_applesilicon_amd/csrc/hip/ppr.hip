@@ -230,6 +230,15 @@ __global__ void k_ppr_sweep(PprArgs A) {
     }
 }
 
+// Rows back to the caller's numbering: dst row v = src row id_of[v].
+__global__ void k_ppr_unpermute(const float* src, float* dst, const uint32_t* id_of, uint32_t n, uint32_t B) {
+    const uint64_t total = (uint64_t)n * B;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = (uint32_t)(i / B), b = (uint32_t)(i % B);
+        dst[i] = src[(size_t)id_of[v] * B + b];
+    }
+}
+
 // One workgroup: column sums of the sweep's workgroup partials, in workgroup order.
 __global__ void k_ppr_reduce(const double* partial, uint32_t blocks, uint32_t B, double* dang_out, double* delta_out) {
     for (uint32_t col = threadIdx.x; col <= B; col += blockDim.x) {
@@ -254,7 +263,7 @@ uint32_t pow2_at_least(uint32_t x) {
 
 // Heavy in-lists of the given threshold, cut into chunks (host, once per threshold).
 void ppr_build_heavy(PprState& S, const PprView& V, uint32_t heavy_min) {
-    if (S.heavy_built && S.heavy_min == heavy_min) return;
+    if (S.heavy_built && S.heavy_min == heavy_min && S.heavy_space == V.space) return;
     std::vector<uint2> hv, ch;
     if (heavy_min != kPprNoHeavy) {
         for (uint32_t v = 0; v < V.n; ++v) {
@@ -276,6 +285,7 @@ void ppr_build_heavy(PprState& S, const PprView& V, uint32_t heavy_min) {
     S.n_heavy = (uint32_t)hv.size() - 1;
     S.n_chunks = (uint32_t)ch.size();
     S.heavy_min = heavy_min;
+    S.heavy_space = V.space;
     S.heavy_built = true;
 }
 
@@ -313,8 +323,7 @@ void ppr_launch(const PprArgs& A, uint32_t grid, uint32_t cgrid, uint32_t tg, hi
 
 }  // namespace
 
-ggpu_pr_result ppr_run(PrEngine& e, const uint32_t* seeds, uint32_t B, const ggpu_pr_opts& o) {
-    PprView V = ppr_view(e);
+ggpu_pr_result ppr_run(PrEngine& e, const uint32_t* seeds_in, uint32_t B, const ggpu_pr_opts& o) {
     PprState& S = ppr_state(e);
     if (B == 0 || B > kPprMaxB) throw ArgError("PPR block width must be in [1, 256]");
     if (o.tg < 64 || o.tg > 1024 || o.tg % 64) throw ArgError("workgroup size must be a multiple of 64 in [64, 1024]");
@@ -322,9 +331,13 @@ ggpu_pr_result ppr_run(PrEngine& e, const uint32_t* seeds, uint32_t B, const ggp
     if (o.large_min < 64) throw ArgError("PPR large_min (heavy in-degree and chunk length) must be >= 64");
     if (o.medium_lanes == 0 || o.medium_lanes > 64 || (o.medium_lanes & (o.medium_lanes - 1)))
         throw ArgError("medium_lanes must be a power of two in [1, 64]");
+    const uint32_t n = ppr_view(e, GGPU_PR_ORDER_GIVEN).n;
     for (uint32_t b = 0; b < B; ++b)
-        if (seeds[b] >= V.n) throw ArgError("seed out of range");
-    const uint32_t n = V.n;
+        if (seeds_in[b] >= n) throw ArgError("seed out of range");
+    const PprView V = ppr_view(e, o.order);
+    std::vector<uint32_t> seeds(seeds_in, seeds_in + B);
+    if (V.id_of)
+        for (auto& x : seeds) x = V.id_of[x];
     const size_t cells = (size_t)n * B;
     hipStream_t s = V.ctx->stream;
     const uint32_t Bw = std::min<uint32_t>(pow2_at_least(B), kWave);
@@ -347,7 +360,7 @@ ggpu_pr_result ppr_run(PrEngine& e, const uint32_t* seeds, uint32_t B, const ggp
     S.B = B;
     std::vector<double> dang0(B);
     for (uint32_t b = 0; b < B; ++b) dang0[b] = V.od_host[seeds[b]] == 0 ? 1.0 : 0.0;
-    GGPU_HIP_TRY(hipMemcpyAsync(S.seeds.p, seeds, 4ull * B, hipMemcpyHostToDevice, s));
+    GGPU_HIP_TRY(hipMemcpyAsync(S.seeds.p, seeds.data(), 4ull * B, hipMemcpyHostToDevice, s));
     GGPU_HIP_TRY(hipMemcpyAsync(S.dang.p, dang0.data(), 8ull * B, hipMemcpyHostToDevice, s));
     GGPU_HIP_TRY(hipEventRecord(S.ev[0], s));
     const uint32_t ib = (uint32_t)std::min<uint64_t>((cells + 255) / 256, 8192);
@@ -415,6 +428,7 @@ ggpu_pr_result ppr_run(PrEngine& e, const uint32_t* seeds, uint32_t B, const ggp
     res.gpu_ms = ms;
     res.iters = k;
     S.slot = k & 1;
+    S.order = o.order;
     S.have = true;
     return res;
 }
@@ -431,26 +445,33 @@ void ppr_autotune(PrEngine& e, const uint32_t* seeds, uint32_t B, int iters, ggp
     for (uint32_t l = Bw; l <= kWave; l <<= 1) lanes.push_back(l);
     const uint32_t heavy[] = {64, 128, 256, 1024};
     const uint32_t tgs[] = {256, 1024};
-    ggpu_pr_opts t = o;
-    t.tol = 0;
-    t.iters = iters;
+    std::vector<ggpu_pr_opts> cands;
+    for (uint32_t ord : {GGPU_PR_ORDER_GIVEN, GGPU_PR_ORDER_DEGREE})
+        for (uint32_t h : heavy)
+            for (uint32_t l : lanes)
+                for (uint32_t tg : tgs) {
+                    ggpu_pr_opts t = o;
+                    t.tol = 0;
+                    t.iters = iters;
+                    t.order = ord;
+                    t.large_min = h;
+                    t.medium_lanes = l;
+                    t.tg = tg;
+                    cands.push_back(t);
+                }
     double best = -1;
     ggpu_pr_opts pick = o;
-    for (uint32_t h : heavy)
-        for (uint32_t l : lanes)
-            for (uint32_t tg : tgs) {
-                t.large_min = h;
-                t.medium_lanes = l;
-                t.tg = tg;
-                ggpu_pr_opts w = t;
-                w.iters = 1;
-                ppr_run(e, seeds, B, w);  // first run of a shape: buffers, heavy lists
-                const double ms = ppr_run(e, seeds, B, t).gpu_ms;
-                if (best < 0 || ms < best) {
-                    best = ms;
-                    pick = t;
-                }
-            }
+    for (const ggpu_pr_opts& t : cands) {
+        ggpu_pr_opts w = t;
+        w.iters = 1;
+        ppr_run(e, seeds, B, w);  // first run of a shape: buffers, heavy lists
+        const double ms = ppr_run(e, seeds, B, t).gpu_ms;
+        if (best < 0 || ms < best) {
+            best = ms;
+            pick = t;
+        }
+    }
+    o.order = pick.order;
     o.large_min = pick.large_min;
     o.medium_lanes = pick.medium_lanes;
     o.tg = pick.tg;
@@ -459,9 +480,23 @@ void ppr_autotune(PrEngine& e, const uint32_t* seeds, uint32_t B, int iters, ggp
 void ppr_fetch(PrEngine& e, float* out) {
     PprState& S = ppr_state(e);
     if (!S.have) throw ArgError("no PPR result yet: call ggpu_ppr_run first", GGPU_ESTATE);
-    PprView V = ppr_view(e);
+    const PprView V = ppr_view(e, S.order);
     const size_t cells = (size_t)V.n * S.B;
-    GGPU_HIP_TRY(hipMemcpy(out, S.r.p + (size_t)S.slot * cells, 4 * cells, hipMemcpyDeviceToHost));
+    const float* src = S.r.p + (size_t)S.slot * cells;
+    if (V.id_of) {
+        if (!S.id_of_ready) {
+            S.id_of.alloc(std::max(V.n, 1u));
+            GGPU_HIP_TRY(hipMemcpy(S.id_of.p, V.id_of, 4ull * V.n, hipMemcpyHostToDevice));
+            S.id_of_ready = true;
+        }
+        float* dst = S.r.p + (size_t)(S.slot ^ 1) * cells;
+        const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((cells + 255) / 256, 8192));
+        k_ppr_unpermute<<<grid, 256, 0, V.ctx->stream>>>(src, dst, S.id_of.p, V.n, S.B);
+        GGPU_HIP_TRY(hipGetLastError());
+        GGPU_HIP_TRY(hipStreamSynchronize(V.ctx->stream));
+        src = dst;
+    }
+    GGPU_HIP_TRY(hipMemcpy(out, src, 4 * cells, hipMemcpyDeviceToHost));
 }
 
 }  // namespace ggpu

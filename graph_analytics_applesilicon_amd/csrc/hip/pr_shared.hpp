@@ -16,6 +16,8 @@ struct PprView {
     const uint32_t* od;
     const uint32_t* od_host;
     const uint32_t* rp_host;
+    const uint32_t* id_of;  // caller id -> id in this numbering; null: identity
+    int space;              // numbering the arrays are in
 };
 
 struct PprState {
@@ -28,7 +30,12 @@ struct PprState {
     DBuf<uint2> chunks, heavy;
     DBuf<float2> cpart;
     uint32_t heavy_min = 0, n_chunks = 0, n_heavy = 0;
+    int heavy_space = -1;
     bool heavy_built = false;
+    // device copy of the degree numbering's id map, for the fetch
+    DBuf<uint32_t> id_of;
+    bool id_of_ready = false;
+    uint32_t order = 0;  // GGPU_PR_ORDER_* numbering of the block held
     int occ_tg = 0, occ_sweep = 0, occ_chunks = 0;  // resident workgroups per CU at occ_tg
     std::vector<hipEvent_t> ev;
     uint32_t B = 0;
@@ -40,7 +47,7 @@ struct PprState {
 };
 
 struct PrEngine;
-PprView ppr_view(PrEngine& e);
+PprView ppr_view(PrEngine& e, uint32_t order);
 PprState& ppr_state(PrEngine& e);
 ggpu_pr_result ppr_run(PrEngine& e, const uint32_t* seeds, uint32_t B, const ggpu_pr_opts& o);
 void ppr_autotune(PrEngine& e, const uint32_t* seeds, uint32_t B, int iters, ggpu_pr_opts& o);

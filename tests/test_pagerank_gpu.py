@@ -153,9 +153,10 @@ def test_pr_errors_are_reported():
 
 # PPR work assignments: the defaults (heavy in-lists >= 256 entries in chunks,
 # a wave per light vertex), every list of >= 64 entries chunked with lane
-# groups of 1 and 16, and no chunking at all.
+# groups of 1 and 16, no chunking at all, and the degree-ordered numbering.
 PPR_ASSIGN = [dict(), dict(large_min=64, medium_lanes=1, tg=64), dict(large_min=64, medium_lanes=16, tg=1024),
-              dict(large_min=0xFFFFFFFF, medium_lanes=64, tg=256)]
+              dict(large_min=0xFFFFFFFF, medium_lanes=64, tg=256),
+              dict(order=G.PR_ORDER_DEGREE, large_min=64, medium_lanes=4, tg=256)]
 
 
 def _ppr_check(tag, blk, refs, tol=2e-5):
@@ -192,7 +193,8 @@ def test_ppr_hub_chunks():
     for B in (3, 130):
         seeds = np.array([hub] + [1 + 997 * i % (g.n - 1) for i in range(B - 1)], dtype=np.uint32)
         refs = [O.ppr(t.row_ptr, t.col_idx, od, int(s), 0.85, 30)[0] for s in seeds]
-        for kw in (dict(large_min=1024), dict(large_min=1000, medium_lanes=4), dict(large_min=0xFFFFFFFF)):
+        for kw in (dict(large_min=1024), dict(large_min=1000, medium_lanes=4), dict(large_min=0xFFFFFFFF),
+                   dict(large_min=1024, order=G.PR_ORDER_DEGREE)):
             pr.run_ppr(seeds, G.pr_options(iters=30, **kw))
             _ppr_check(f"star B={B} {kw}", pr.ppr_block(), refs)
 
@@ -203,11 +205,12 @@ def test_ppr_autotune_and_errors():
     seeds = np.arange(0, 8 * 97, 97, dtype=np.uint32)
     o = pr.autotune_ppr(seeds, iters=2, opts=G.pr_options(iters=40))
     assert o.large_min in (64, 128, 256, 1024) and o.medium_lanes in (8, 16, 32, 64) and o.tg in (256, 1024)
+    assert o.order in (G.PR_ORDER_GIVEN, G.PR_ORDER_DEGREE)
     assert o.iters == 40
     pr.run_ppr(seeds, o)
     refs = [O.ppr(t.row_ptr, t.col_idx, od, int(s), 0.85, 40)[0] for s in seeds]
     _ppr_check(f"autotuned {o.large_min} {o.medium_lanes} {o.tg}", pr.ppr_block(), refs)
-    for kw in (dict(large_min=63), dict(medium_lanes=3), dict(medium_lanes=128), dict(tg=96)):
+    for kw in (dict(large_min=63), dict(medium_lanes=3), dict(medium_lanes=128), dict(tg=96), dict(order=2)):
         with pytest.raises(ValueError):
             pr.run_ppr(seeds, G.pr_options(**kw))
     with pytest.raises(ValueError):
