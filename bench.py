@@ -76,11 +76,15 @@ def main():
                     help="graph of the PageRank part (default: the scale-23 heavy-skew RMAT)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="auto", choices=["auto", "batched", "sequential"])
-    ap.add_argument("--only", default="all", choices=["all", "bfs", "pagerank", "loops"],
+    ap.add_argument("--only", default="all", choices=["all", "bfs", "pagerank", "loops", "ppr"],
                     help="profiling runs: just one part (the JSON line then carries that part)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 PageRank reference (profiling runs)")
     ap.add_argument("--no-loops", action="store_true", help="skip the level-loop comparison on the grid graph")
+    ap.add_argument("--no-ppr", action="store_true", help="skip the personalized-PageRank block")
+    ap.add_argument("--ppr-seeds", type=int, default=64, help="columns of the personalized-PageRank block")
+    ap.add_argument("--ppr-assign", default=None, metavar="ORDER:LARGE_MIN:LANES:TG",
+                    help="fixed PPR work assignment instead of the autotuner's (profiling runs), e.g. degree:1024:64:1024")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -95,7 +99,7 @@ def main():
 
     ctx = G.default_context()
     name, cus, _ = ctx.info()
-    bfs_res = pr_res = cpu = loops = None
+    bfs_res = pr_res = cpu = loops = ppr = None
     if args.only in ("all", "bfs"):
         bfs_res = bfs_part(G, O, args, ctx)
     if args.only == "loops" or (args.only == "all" and not args.no_loops):
@@ -104,6 +108,9 @@ def main():
         pg = G.Graph.named(args.pr_graph)
         pr_res = pagerank_part(G, O, pg, ctx, args.pr_graph, with_ref=not args.no_ref)
         pr_res["graph"] = args.pr_graph
+    if args.only == "ppr" or (args.only == "all" and not args.no_ppr):
+        pg = pg if pr_res is not None else G.Graph.named(args.pr_graph)
+        ppr = ppr_part(G, pg, ctx, args.pr_graph, args.ppr_seeds, assign=args.ppr_assign)
     if bfs_res is not None and not args.no_cpu:
         cpu = cpu_baseline(O, bfs_res.pop("_graph"), bfs_res.pop("_sources"), bfs_res["comp_edges"])
     if bfs_res is not None:
@@ -139,6 +146,15 @@ def main():
         if pr_res is not None:
             line["pagerank_roofline"] = pr_res.pop("roofline")
             line["pagerank"] = pr_res
+        if ppr is not None:
+            line["ppr"] = ppr
+    elif ppr is not None and pr_res is None:
+        line = {"metric": f"personalized PageRank block ms per sweep ({args.ppr_seeds} seeds)",
+                "value": ppr["ms_per_sweep"], "unit": "ms", "n_gpus": 1, "steps": ppr["sweeps"], "warmup": 1,
+                "ms_per_step": ppr["ms_per_sweep"], "higher_is_better": False, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                "config": {"workload": ppr["workload"], "device": name}, "roofline": ppr.pop("roofline"),
+                "cpu_baseline": None, "ppr": ppr}
     elif pr_res is None:
         line = {"metric": "BFS us per level on the grid graph (gpu_do)", "value": loops["persistent"]["us_per_level"],
                 "unit": "us/level", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -332,6 +348,53 @@ def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
     log(f"pagerank: certified Jacobi {res['jacobi_certified']['ms']:.2f} ms "
         f"({res['jacobi_certified']['iters']} it), GS {res['gs_tol_1e-6']['ms']:.2f} ms "
         f"({res['gs_tol_1e-6']['iters']} it), sweep {per_sweep_ms * 1e3:.1f} us = {ach:.0f} GB/s")
+    return res
+
+
+def ppr_part(G, g, ctx, g_name, B=64, sweeps=10, assign=None):
+    """Personalized PageRank block of B seeds (pickSources, seed 5): work
+    assignment and numbering from the autotuner, then the per-sweep time as the
+    difference of a 2*sweeps and a sweeps run (the block's initialisation
+    cancels). Every column must keep mass 1."""
+    n, m = g.n, g.m
+    od = g.degrees()
+    t = g if not g.directed else g.transpose()
+    pr = G.GpuPageRank(t, od, ctx)
+    seeds = np.asarray(g.pick_sources(B, 5), dtype=np.uint32)
+    t0 = time.perf_counter()
+    if assign:
+        order, large_min, lanes, tg = assign.split(":")
+        o = G.pr_options(order=G.PR_ORDER_DEGREE if order == "degree" else G.PR_ORDER_GIVEN,
+                         large_min=int(large_min), medium_lanes=int(lanes), tg=int(tg))
+    else:
+        o = pr.autotune_ppr(seeds, iters=3)
+    tune_s = time.perf_counter() - t0
+    o.tol = 0
+    o.iters = sweeps
+    r1 = pr.run_ppr(seeds, o)
+    o.iters = 2 * sweeps
+    r2 = pr.run_ppr(seeds, o)
+    per = (r2.gpu_ms - r1.gpu_ms) / sweeps
+    blk = pr.ppr_block()
+    sums = blk.sum(axis=0, dtype=np.float64)
+    mass_err = float(np.abs(sums - 1.0).max())
+    seed_min = float(blk[seeds, np.arange(B)].min())
+    del blk
+    # rows of the in-CSR, the lists, out-degrees, one row gather of 4B bytes per
+    # in-edge, rank and contribution rows written
+    bytes_sweep = 4 * (n + 1) + 4 * m + 4 * n + 4 * B * m + 8 * B * n
+    ach = bytes_sweep / (per / 1e3) / 1e9
+    res = {"workload": f"{g_name}: RMAT scale 23, ef 16, a/b/c .60/.17/.17; {B} pickSources seeds (seed 5)",
+           "seeds": B, "sweeps": sweeps, "ms_per_sweep": per, "gpu_ms_2x_sweeps": r2.gpu_ms,
+           "autotune_s": tune_s, "assignment": {"order": "degree" if o.order == G.PR_ORDER_DEGREE else "given",
+                                               "large_min": o.large_min, "medium_lanes": o.medium_lanes, "tg": o.tg},
+           "max_column_mass_error": mass_err, "min_seed_rank": seed_min,
+           "roofline": {"kernel": "k_ppr_chunks + k_ppr_sweep + k_ppr_reduce per sweep", "bound": "hbm",
+                        "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                        "traffic": measured_traffic("k_ppr_chunks+k_ppr_sweep", g_name), "bytes_per_launch": bytes_sweep,
+                        "avg_launch_ms": per}}
+    log(f"ppr: {B} seeds, {per:.3f} ms/sweep = {ach:.0f} GB/s ({ach / HBM_PEAK_GBS * 100:.1f}% of HBM), "
+        f"assignment {res['assignment']}, autotune {tune_s:.1f}s, column mass error {mass_err:.2e}")
     return res
 
 

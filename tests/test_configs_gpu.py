@@ -6,7 +6,11 @@
 * rmat22_c2 (scale 22, ef 8, permuted): single-source gpu_do, the 64-source
   batch under the learned policy (depths of the checked sources bit for bit),
   gpu_pull_gs to tol 1e-6 judged against the converged FP64 ranks.
-* rmat23_c5 (scale 23, ef 16, a = .60): Jacobi certified to L1 < 1e-4.
+* rmat23_c5 (scale 23, ef 16, a = .60): Jacobi certified to L1 < 1e-4; the
+  64-seed personalized-PageRank block: every column keeps mass 1, two work
+  assignments (given numbering, no chunks / degree numbering, 1024-entry
+  chunks, lane groups) agree column by column, and one column matches the
+  oracle's FP64 PPR after 6 sweeps.
 
 Checkers: the oracle's serial BFS and its cpu_do level restatement; the
 FP64 PageRank of the compiled reference (oracle/_ref, OpenMP) when it is
@@ -96,3 +100,23 @@ def test_rmat23_c5_certified_jacobi():
         l1, rel, s = O.check_ranks(conv, pr.ranks())
         assert not res.floor_stop and res.last_delta < tol_cert, (layout, res.last_delta)
         assert l1 < 1e-4 and rel < 1e-3 and abs(s - 1) < 1e-4, (layout, l1, rel, s)
+
+
+def test_rmat23_c5_ppr_block():
+    g = G.Graph.named("rmat23_c5")
+    od = g.degrees()
+    B = 64
+    seeds = np.asarray(g.pick_sources(B, 5), dtype=np.uint32)
+    pr = G.GpuPageRank(g, od)
+    sweeps = 6
+    pr.run_ppr(seeds, G.pr_options(iters=sweeps, order=G.PR_ORDER_DEGREE, large_min=1024, tg=1024))
+    a = pr.ppr_block()
+    sums = a.sum(axis=0, dtype=np.float64)
+    assert np.abs(sums - 1).max() < 1e-4, np.abs(sums - 1).max()
+    ref, _, _ = O.ppr(g.row_ptr, g.col_idx, od, int(seeds[0]), 0.85, sweeps)
+    err = float(np.abs(a[:, 0].astype(np.float64) - ref).sum())
+    assert err < 1e-5, err
+    pr.run_ppr(seeds, G.pr_options(iters=sweeps, order=G.PR_ORDER_GIVEN, large_min=0xFFFFFFFF, tg=256))
+    b = pr.ppr_block()
+    diff = np.abs(a.astype(np.float64) - b).sum(axis=0)
+    assert diff.max() < 1e-5, diff.max()
