@@ -44,6 +44,7 @@ int ggpu_ctx_create(int device, ggpu_ctx** out) {
         if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
             throw ArgError(std::string("this backend is built for gfx950 only; device is ") + prop.gcnArchName);
         c->name = prop.name;
+        if (c->name.empty()) c->name = std::string("AMD ") + prop.gcnArchName;  // marketing name not reported
         c->cus = prop.multiProcessorCount;
         c->hbm = prop.totalGlobalMem;
         GGPU_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
