@@ -593,13 +593,17 @@ __device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool
                 } else {
                     continue;
                 }
-                D.dist[(size_t)(s0 + b) * D.n + v] = d;
+                // streamed out: the depth rows are not read again by the traversal
+                __builtin_nontemporal_store(d, &D.dist[(size_t)(s0 + b) * D.n + v]);
             }
         }
         for (uint32_t i = 0; i < cnt; ++i) {
             const size_t slot = (base + i) % D.C;
+            // F of a vertex outside level i's frontier is already zero
+            if ((levels >> i) & 1ull) {
 #pragma unroll
-            for (int w = 0; w < W; ++w) ms_clear<W>(&D.F[(slot * D.n + v) * W + w], final);
+                for (int w = 0; w < W; ++w) ms_clear<W>(&D.F[(slot * D.n + v) * W + w], final);
+            }
             // the whole wave has read the frontier word of its 64-vertex group
             if (lane == 0) ms_clear<W>(&D.fb[slot * D.nb + wd], final);
         }
