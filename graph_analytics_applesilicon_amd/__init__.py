@@ -32,16 +32,17 @@ INF = 0xFFFFFFFF
 OK, EINVAL, EDEVICE, EIO, ENOMEM, ESTATE, ETIMEOUT = range(7)
 FLAG_DIRECTED, FLAG_SORTED, FLAG_DEDUP = 1, 2, 4
 
-BFS_EDGE, BFS_BINNED, BFS_DO, BFS_HYBRID = 0, 1, 2, 3
-BFS_VARIANTS = {"gpu_edge": BFS_EDGE, "gpu_binned": BFS_BINNED, "gpu_do": BFS_DO, "hybrid": BFS_HYBRID}
+BFS_EDGE, BFS_BINNED, BFS_DO, BFS_HYBRID, BFS_VERTEX, BFS_NAIVE = 0, 1, 2, 3, 4, 5
+BFS_VARIANTS = {"gpu_edge": BFS_EDGE, "gpu_binned": BFS_BINNED, "gpu_do": BFS_DO, "hybrid": BFS_HYBRID,
+                "gpu_vertex": BFS_VERTEX, "gpu_naive": BFS_NAIVE}
 LOOP_HOST, LOOP_GRAPH, LOOP_PERSISTENT = 0, 1, 2
 POLICY_HEURISTIC, POLICY_LEARNED, POLICY_SCRIPTED = 0, 1, 2
 ACTION_TD_EDGE, ACTION_TD_BINNED, ACTION_BU = 0, 1, 2
-PR_SCALAR, PR_VECTOR, PR_ADAPTIVE, PR_GS = 1, 2, 3, 4
+PR_SCALAR, PR_VECTOR, PR_ADAPTIVE, PR_GS, PR_PUSH = 1, 2, 3, 4, 5
 PR_LAYOUT_STREAM, PR_LAYOUT_BINS, PR_LAYOUT_HOT = 0, 1, 2
 PR_ORDER_GIVEN, PR_ORDER_DEGREE = 0, 1
 PR_VARIANTS = {"gpu_pull_scalar": PR_SCALAR, "gpu_pull_vector": PR_VECTOR,
-               "gpu_pull_adaptive": PR_ADAPTIVE, "gpu_pull_gs": PR_GS}
+               "gpu_pull_adaptive": PR_ADAPTIVE, "gpu_pull_gs": PR_GS, "gpu_push": PR_PUSH}
 MLP_PARAMS = 1347
 POLICY_SINGLE, POLICY_BATCHED = 0, 1
 

@@ -16,9 +16,10 @@
 //          [--policy-file W.f32] [--warmup 2] [--trials 5] [--validate] [--csv out.csv]
 //   gbench --graph X --save-graph out.bin
 //
-// BFS variants: cpu_serial cpu_omp cpu_do (host library) and gpu_edge
-// gpu_binned gpu_do hybrid (HIP backend). PageRank: cpu_serial cpu_omp and
-// gpu_pull_scalar gpu_pull_vector gpu_pull_adaptive gpu_pull_gs. msbfs (no
+// BFS variants: cpu_serial cpu_omp cpu_do (host library) and gpu_vertex
+// gpu_naive gpu_binned gpu_edge gpu_do hybrid (HIP backend). PageRank:
+// cpu_serial cpu_omp and gpu_push gpu_pull_scalar gpu_pull_vector
+// gpu_pull_adaptive gpu_pull_gs. msbfs (no
 // reference counterpart) runs the sources as batches of --batch in one launch
 // each. Exit codes as in the reference: 0 ok, 1 failure / validation
 // mismatch, 2 usage (unknown variant or algorithm, bad source).
@@ -256,7 +257,8 @@ void set_policy(const Args& a, const Device& d, ggpu_bfs_opts& o, int slot) {
 }
 
 // ------------------------------------------------------------------- BFS
-const char* kBfsVariants[] = {"cpu_serial", "cpu_omp", "cpu_do", "gpu_edge", "gpu_binned", "gpu_do", "hybrid"};
+const char* kBfsVariants[] = {"cpu_serial", "cpu_omp", "cpu_do",   "gpu_vertex", "gpu_naive",
+                              "gpu_binned", "gpu_edge", "gpu_do",   "hybrid"};
 
 int run_bfs(const Args& a, const Graph& stored, const std::string& dataset) {
     const std::string variant = a.get("variant", "cpu_serial");
@@ -540,7 +542,7 @@ int run_msbfs(const Args& a, const Graph& stored, const std::string& dataset) {
 }
 
 // -------------------------------------------------------------- PageRank
-const char* kPrVariants[] = {"cpu_serial", "cpu_omp", "gpu_pull_scalar", "gpu_pull_vector", "gpu_pull_adaptive",
+const char* kPrVariants[] = {"cpu_serial", "cpu_omp", "gpu_push", "gpu_pull_scalar", "gpu_pull_vector", "gpu_pull_adaptive",
                              "gpu_pull_gs"};
 
 // cpu::gsNoWorseThanJacobi (src/cpu/pagerank_cpu.hpp:80-84)

@@ -27,6 +27,58 @@ __global__ void k_init(BfsDev d, uint32_t src, uint32_t src_bin) {
         if (d.bins[0][src_bin]) d.bins[0][src_bin][0] = src;
     }
 }
+// gpu_vertex level (bfs.metal:45-79): a lane per vertex; vertices at depth
+// `level` store level + 1 into their unreached neighbours. Concurrent stores
+// to one vertex write the same value, so they are plain stores; a vertex
+// stored twice is counted twice in S_FOUND, which only decides whether
+// another level runs.
+__global__ void k_vertex_level(BfsDev d, uint32_t level, unsigned long long* slot) {
+    unsigned long long fr = 0, ed = 0, fo = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < d.n; v += gridDim.x * blockDim.x) {
+        if (d.dist[v] != level) continue;
+        const uint32_t e0 = d.rp[v], e1 = d.rp[v + 1];
+        ++fr;
+        ed += e1 - e0;
+        for (uint32_t e = e0; e < e1; ++e) {
+            const uint32_t u = d.ci[e];
+            if (d.dist[u] == GGPU_INF) {
+                d.dist[u] = level + 1;
+                ++fo;
+            }
+        }
+    }
+    const int words[3] = {S_FRONTIER, S_EXAMINED, S_FOUND};
+    const unsigned long long vals[3] = {fr, ed, fo};
+    block_flush<3>(slot, words, vals);
+}
+
+// gpu_naive level (bfs.metal:95-126): a lane per queued vertex walks its whole
+// list; a neighbour is claimed by compare-and-swap INF -> level + 1 (after a
+// plain load filters visited ones) and every claim appends itself to the next
+// queue with its own atomic on the tail, so the queue is duplicate-free and
+// the tail is the exact count of discovered vertices.
+__global__ void k_naive_level(BfsDev d, const uint32_t* q, uint32_t* nq, uint32_t size, uint32_t level,
+                              unsigned long long* slot) {
+    unsigned long long fr = 0, ed = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < size; i += gridDim.x * blockDim.x) {
+        const uint32_t v = q[i];
+        const uint32_t e0 = d.rp[v], e1 = d.rp[v + 1];
+        ++fr;
+        ed += e1 - e0;
+        for (uint32_t e = e0; e < e1; ++e) {
+            const uint32_t u = d.ci[e];
+            if (d.dist[u] != GGPU_INF) continue;
+            if (atomicCAS(&d.dist[u], GGPU_INF, level + 1) == GGPU_INF) {
+                const unsigned long long pos = atomicAdd(&d.conv[kTailQueue], 1ull);
+                nq[pos] = u;
+            }
+        }
+    }
+    const int words[2] = {S_FRONTIER, S_EXAMINED};
+    const unsigned long long vals[2] = {fr, ed};
+    block_flush<2>(slot, words, vals);
+}
+
 __global__ void k_scan_tiles(uint32_t* vals, uint32_t size, uint32_t* tile) {
     ph_scan_tiles(vals, size, tile, blockIdx.x, gridDim.x);
 }
@@ -252,6 +304,67 @@ void BfsEngine::host_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, std
     }
 }
 
+// gpu_vertex / gpu_naive (runVertex / runNaive, bfs_gpu.mm:115-175): one
+// top-down launch per level; the level with nothing discovered is the last
+// (it is recorded, as the reference records it).
+void BfsEngine::simple_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, std::vector<ggpu_level>* trace) {
+    const uint32_t tg = o.tg;
+    const bool timing = o.time_levels != 0;
+    hipStream_t s = ctx->stream;
+    BfsDev d = dev();
+    k_init<<<blocks_for(n, 256, grid_cap), 256, 0, s>>>(d, src, 0);
+    GGPU_HIP_TRY(hipGetLastError());
+    last_levels = 0;
+    last_gpu_ms = 0;
+    size_t ev = 0;
+    uint32_t size = 1, cur = 0;
+    for (uint32_t level = 0;; ++level) {
+        const auto t0 = std::chrono::steady_clock::now();
+        unsigned long long* slot = stats.p;
+        const size_t words = kSlotWords + kConvWords;
+        GGPU_HIP_TRY(hipMemsetAsync(slot, 0, words * sizeof(unsigned long long), s));
+        if (timing) GGPU_HIP_TRY(hipEventRecord(event(ev), s));
+        if (variant == GGPU_BFS_VERTEX)
+            k_vertex_level<<<blocks_for(n, tg, grid_cap), tg, 0, s>>>(d, level, slot);
+        else
+            k_naive_level<<<blocks_for(size, tg, grid_cap), tg, 0, s>>>(d, d.q[cur], d.q[cur ^ 1], size, level, slot);
+        GGPU_HIP_TRY(hipGetLastError());
+        if (timing) GGPU_HIP_TRY(hipEventRecord(event(ev + 1), s));
+        ev += timing ? 2 : 0;
+        auto* pin = (unsigned long long*)ctx_pinned(ctx, 4096);
+        GGPU_HIP_TRY(hipMemcpyAsync(pin, slot, words * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        GGPU_HIP_TRY(hipStreamSynchronize(s));
+        unsigned long long sum[kStatWords] = {0};
+        for (int t = 0; t < kStripes; ++t)
+            for (int k = 0; k < kStatWords; ++k) sum[k] += pin[t * kStatWords + k];
+        const unsigned long long found = variant == GGPU_BFS_VERTEX ? sum[S_FOUND] : pin[kSlotWords + kTailQueue];
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (trace) {
+            ggpu_level L{};
+            L.level = level;
+            L.action = GGPU_ACTION_TD_EDGE;
+            L.frontier = sum[S_FRONTIER];
+            L.edges = sum[S_EXAMINED];
+            L.frontier_edges = sum[S_EXAMINED];
+            L.direction = 'T';
+            L.device = 'G';
+            L.time_ms = ms;
+            trace->push_back(L);
+        }
+        ++last_levels;
+        if (found == 0) break;
+        size = (uint32_t)found;
+        cur ^= 1;
+    }
+    if (timing) {
+        for (size_t i = 0; i + 1 < ev; i += 2) {
+            float ms = 0;
+            GGPU_HIP_TRY(hipEventElapsedTime(&ms, events[i], events[i + 1]));
+            last_gpu_ms += ms;
+        }
+    }
+}
+
 // Enqueues one GPU level (conversion if the frontier is in another format,
 // then the action's kernels). Grids are sized exactly from the host state.
 void BfsEngine::launch_level(BfsDev& d, LoopState& st, const ggpu_bfs_opts& o, unsigned long long* slot, uint32_t tg) {
@@ -426,6 +539,8 @@ int ggpu_bfs_parse_variant(const char* name, int* variant) {
     else if (s == "gpu_binned") *variant = GGPU_BFS_BINNED;
     else if (s == "gpu_do") *variant = GGPU_BFS_DO;
     else if (s == "hybrid") *variant = GGPU_BFS_HYBRID;
+    else if (s == "gpu_vertex") *variant = GGPU_BFS_VERTEX;
+    else if (s == "gpu_naive") *variant = GGPU_BFS_NAIVE;
     else return GGPU_EINVAL;
     return GGPU_OK;
 }
@@ -457,7 +572,7 @@ int ggpu_bfs_run(ggpu_bfs* b, int variant, uint32_t source, const ggpu_bfs_opts*
     if (opt) o = *opt;
     return guarded(ctx, [&] {
         BfsEngine& e = *b->eng;
-        if (variant < GGPU_BFS_EDGE || variant > GGPU_BFS_HYBRID) throw ArgError("unknown BFS variant");
+        if (variant < GGPU_BFS_EDGE || variant > GGPU_BFS_NAIVE) throw ArgError("unknown BFS variant");
         if (source >= e.n) throw ArgError("source out of range");
         check_tg(o.tg);
         if (o.ept == 0) throw ArgError("ept must be >= 1");
@@ -468,10 +583,14 @@ int ggpu_bfs_run(ggpu_bfs* b, int variant, uint32_t source, const ggpu_bfs_opts*
         if (o.loop > GGPU_LOOP_PERSISTENT) throw ArgError("unknown loop mode");
         if (variant == GGPU_BFS_HYBRID && (o.loop != GGPU_LOOP_HOST || o.policy != GGPU_POLICY_HEURISTIC))
             throw ArgError("hybrid runs with the host loop and the heuristic policy");
+        const bool simple = variant == GGPU_BFS_VERTEX || variant == GGPU_BFS_NAIVE;
+        if (simple && (o.loop != GGPU_LOOP_HOST || o.policy != GGPU_POLICY_HEURISTIC))
+            throw ArgError("gpu_vertex and gpu_naive run top-down with the host loop and the heuristic policy");
         GGPU_HIP_TRY(hipSetDevice(ctx->device));
         std::vector<ggpu_level> tr;
         std::vector<ggpu_level>* trp = (trace || trace_len) ? &tr : nullptr;
-        if (o.loop == GGPU_LOOP_HOST) e.host_loop(variant, source, o, trp);
+        if (simple) e.simple_loop(variant, source, o, trp);
+        else if (o.loop == GGPU_LOOP_HOST) e.host_loop(variant, source, o, trp);
         else e.device_loop(variant, source, o, trp);
         if (trace) std::memcpy(trace, tr.data(), std::min(tr.size(), trace_cap) * sizeof(ggpu_level));
         if (trace_len) *trace_len = tr.size();

@@ -221,6 +221,8 @@ struct BfsEngine {
     HybridMirror& mirror();
 
     void host_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, std::vector<ggpu_level>* trace);
+    // gpu_vertex / gpu_naive: top-down levels until nothing is discovered
+    void simple_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, std::vector<ggpu_level>* trace);
     void device_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, std::vector<ggpu_level>* trace);
     void launch_level(BfsDev& d, LoopState& st, const ggpu_bfs_opts& o, unsigned long long* slot, uint32_t tg);
     void download_state(LoopState& st, HybridMirror& hm);

@@ -32,6 +32,7 @@ namespace ggpu {
 constexpr uint32_t kSecBlocks = 2048;             // block cap per section (grid-stride beyond)
 constexpr uint32_t kStreamBlocks = 8192;          // block cap of a stream-layout sweep
 constexpr uint32_t kPartialCap = 1u << 17;        // FP64 partial slots per quantity (all launches of a sweep)
+constexpr uint32_t kPushHeavy = 256;              // gpu_push: out-degree from which a wave pushes a source
 
 // ------------------------------------------------------------------ kernels
 __global__ void k_pr_sweep(PrSweep P) { pr_sweep_body(P); }
@@ -55,6 +56,50 @@ __global__ void k_pr_init(float* r, float* contrib, const uint32_t* outdeg, uint
         part.sum += (double)r0;
     }
     write_block_partials(part, partial, cap, blockIdx.x);
+}
+
+// gpu_push (pagerank.metal pr_push / pr_push_finalize): acc[t] gathers the
+// contributions of t's in-neighbours through FP64 atomic adds (the hardware
+// f64 add on this coarse-grained device memory), so the order in which the
+// adds land moves the sum by ~1e-16 relative and needs no compensation: a
+// hub's 1e5 adds of near-equal FP32 terms round the same way every time, and
+// neither an FP32 accumulator nor an FP32 error accumulator beside it keeps
+// that below 1e-6.
+__device__ __forceinline__ void push_add(double* acc, uint32_t t, float x) { unsafeAtomicAdd(&acc[t], (double)x); }
+
+// Sources of out-degree >= kPushHeavy are pushed by a wave each (lanes stride
+// the out-list), the others by a lane each; zero contributions push nothing.
+__global__ void k_pr_push(const uint32_t* orp, const uint32_t* oci, const float* contrib, double* acc, uint32_t n,
+                          const uint32_t* heavy, uint32_t n_heavy) {
+    const uint32_t lane = lane_id(), wpb = blockDim.x / kWave;
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    for (uint32_t h = blockIdx.x * wpb + wid; h < n_heavy; h += gridDim.x * wpb) {
+        const uint32_t u = heavy[h];
+        const float c = contrib[u];
+        if (c == 0.0f) continue;
+        for (uint32_t e = orp[u] + lane; e < orp[u + 1]; e += kWave) push_add(acc, oci[e], c);
+    }
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+        const uint32_t e0 = orp[u], e1 = orp[u + 1];
+        if (e1 - e0 >= kPushHeavy) continue;
+        const float c = contrib[u];
+        if (c == 0.0f) continue;
+        for (uint32_t e = e0; e < e1; ++e) push_add(acc, oci[e], c);
+    }
+}
+
+// r_new = (1-d)/n + d * (acc + dangling/n), stored like the pull sweeps store
+// (contributions, FP64 partials); the accumulators are cleared.
+__global__ void k_pr_push_finish(PrSweep P, double* acc) {
+    const double addd = (1.0 - (double)P.damping) / (double)P.n + (double)P.damping * P.acc_in->dangling / (double)P.n;
+    const float add = (float)addd, d = P.damping, inv_n = 1.0f / (float)P.n;
+    PrPart part;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < P.n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const float val = add + d * (float)acc[v];
+        acc[v] = 0.0;
+        pr_store(P, (uint32_t)v, val, part, inv_n);
+    }
+    write_block_partials(part, P.partial, P.partial_cap, P.partial_base + blockIdx.x);
 }
 
 // After an in-place sweep: r_new *= 1/sum (sum from acc_sweep), L1 change
@@ -109,6 +154,10 @@ struct Space {
     std::unique_ptr<StreamLayout> wstream;              // HOT layout: wave-sized items over all rows
     DBuf<uint32_t> bins;
     uint32_t bin_count[3] = {0, 0, 0};
+    // gpu_push: out-arcs (the transpose of `in`), sources of out-degree >= kPushHeavy
+    DBuf<uint32_t> orp, oci, oheavy;
+    uint32_t n_oheavy = 0;
+    bool out_ready = false;
     uint32_t bin_small = 0, bin_large = 0;
     bool bins_ready = false;
 };
@@ -119,6 +168,7 @@ struct PrEngine {
     uint64_t m;
     Space sp[SP_COUNT];
     DBuf<float> r, contrib;  // 2n each
+    DBuf<double> pacc;       // gpu_push: FP64 accumulators acc[n] (zero between sweeps)
     DBuf<double> partial;
     uint32_t npart = 0;      // partial slots written since the last reduction
     DBuf<PrAcc> acc;         // [0],[1] Jacobi ping-pong / in-place state, [2] in-place sweep totals
@@ -186,6 +236,48 @@ struct PrEngine {
         S.stream.clear();
         S.wstream.reset();
         S.bins_ready = false;
+        S.out_ready = false;
+    }
+
+    // gpu_push's out-arcs of space S: the transpose of its in-CSR.
+    void ensure_out(Space& S) {
+        if (S.out_ready) return;
+        const Csr out = transpose(S.in);
+        S.orp.alloc(n + 1);
+        S.oci.alloc(out.m ? out.m : 1);
+        GGPU_HIP_TRY(hipMemcpy(S.orp.p, out.rp.data(), 4ull * (n + 1), hipMemcpyHostToDevice));
+        if (out.m) GGPU_HIP_TRY(hipMemcpy(S.oci.p, out.ci.data(), 4ull * out.m, hipMemcpyHostToDevice));
+        std::vector<uint32_t> heavy;
+        for (uint32_t v = 0; v < n; ++v)
+            if (out.deg(v) >= kPushHeavy) heavy.push_back(v);
+        S.oheavy.alloc(std::max<size_t>(heavy.size(), 1));
+        if (!heavy.empty())
+            GGPU_HIP_TRY(hipMemcpy(S.oheavy.p, heavy.data(), 4ull * heavy.size(), hipMemcpyHostToDevice));
+        S.n_oheavy = (uint32_t)heavy.size();
+        S.out_ready = true;
+    }
+
+    // One gpu_push sweep: atomics along the out-arcs, then the stores and
+    // FP64 partials of every vertex (the accumulators are cleared for the next).
+    void push_sweep(Space& S, const ggpu_pr_opts& o, int k, bool want_delta) {
+        const int c = k & 1, nx = c ^ 1;
+        PrSweep P = base_sweep(S, o.damping);
+        P.contrib_in = contrib.p + (size_t)c * n;
+        P.contrib_out = contrib.p + (size_t)nx * n;
+        P.r_old = want_delta ? r.p + (size_t)c * n : nullptr;
+        P.r_new = r.p + (size_t)nx * n;
+        P.acc_in = acc.p + c;
+        const uint32_t tg = o.tg;
+        const uint32_t grid = blocks(std::max<uint64_t>(n, (uint64_t)S.n_oheavy * kWave), tg);
+        k_pr_push<<<grid, tg, 0, ctx->stream>>>(S.orp.p, S.oci.p, P.contrib_in, pacc.p, n, S.oheavy.p, S.n_oheavy);
+        GGPU_HIP_TRY(hipGetLastError());
+        const uint32_t g = blocks(n, 256);
+        if (npart + g > kPartialCap) throw ArgError("too many blocks in one sweep");
+        P.partial_base = npart;
+        npart += g;
+        k_pr_push_finish<<<g, 256, 0, ctx->stream>>>(P, pacc.p);
+        GGPU_HIP_TRY(hipGetLastError());
+        reduce(acc.p + nx);
     }
 
     Space& space(int id) {
@@ -484,7 +576,8 @@ struct PrEngine {
 
     ggpu_pr_result run(int variant, const ggpu_pr_opts& o) {
         if (o.tg < 64 || o.tg > 1024 || o.tg % 64) throw ArgError("workgroup size must be a multiple of 64 in [64, 1024]");
-        if (variant < GGPU_PR_SCALAR || variant > GGPU_PR_GS) throw ArgError("unknown PageRank variant");
+        if (variant < GGPU_PR_SCALAR || variant > GGPU_PR_PUSH) throw ArgError("unknown PageRank variant");
+        if (variant == GGPU_PR_PUSH && o.in_place) throw ArgError("gpu_push runs Jacobi sweeps only (in_place = 0)");
         const uint32_t L = o.medium_lanes;
         if (L == 0 || L > 64 || (L & (L - 1))) throw ArgError("medium_lanes must be a power of two in [1, 64]");
         if (o.iters < 0) throw ArgError("iters must be >= 0");
@@ -508,6 +601,11 @@ struct PrEngine {
         const bool tol_mode = o.tol > 0;
         GGPU_HIP_TRY(hipEventRecord(event(0), s));
         npart = 0;
+        if (variant == GGPU_PR_PUSH) {
+            ensure_out(S);
+            pacc.alloc(n);
+            GGPU_HIP_TRY(hipMemsetAsync(pacc.p, 0, 8ull * n, s));
+        }
         const uint32_t gi = blocks(n, 256);
         k_pr_init<<<gi, 256, 0, s>>>(r.p, contrib.p, S.odd.p, n, partial.p, kPartialCap);
         GGPU_HIP_TRY(hipGetLastError());
@@ -523,7 +621,8 @@ struct PrEngine {
                 inplace_sweep(S, variant, o, k);
                 out = acc.p + 0;
             } else {
-                jacobi_sweep(S, variant, o, k, tol_mode);
+                if (variant == GGPU_PR_PUSH) push_sweep(S, o, k, tol_mode);
+                else jacobi_sweep(S, variant, o, k, tol_mode);
                 out = acc.p + ((k + 1) & 1);
             }
             if (!tol_mode) continue;
@@ -664,6 +763,7 @@ int ggpu_pr_parse_variant(const char* name, int* variant) {
     else if (s == "gpu_pull_vector") *variant = GGPU_PR_VECTOR;
     else if (s == "gpu_pull_adaptive") *variant = GGPU_PR_ADAPTIVE;
     else if (s == "gpu_pull_gs") *variant = GGPU_PR_GS;
+    else if (s == "gpu_push") *variant = GGPU_PR_PUSH;
     else return GGPU_EINVAL;
     return GGPU_OK;
 }

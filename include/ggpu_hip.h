@@ -45,6 +45,8 @@ int ggpu_ctx_sync(ggpu_ctx* ctx);
 #define GGPU_BFS_BINNED 1 /* gpu_binned: thread / wave / workgroup per frontier vertex by degree */
 #define GGPU_BFS_DO 2     /* gpu_do:     direction-optimizing (top-down edge-balanced + bitmap bottom-up) */
 #define GGPU_BFS_HYBRID 3 /* hybrid:     gpu_do with levels whose work < hybrid_t on the CPU */
+#define GGPU_BFS_VERTEX 4 /* gpu_vertex: lane per vertex every level, plain stores (bfs.metal:45-79) */
+#define GGPU_BFS_NAIVE 5  /* gpu_naive:  lane per queued vertex, CAS claims, one append atomic per claim (bfs.metal:95-126) */
 
 /* How the level loop is driven (no reference counterpart except HOST). */
 #define GGPU_LOOP_HOST 0       /* one host synchronization per level (the reference protocol) */
@@ -77,7 +79,9 @@ typedef struct ggpu_bfs_opts {
     uint32_t time_levels;   /* HOST loop: 1 = record per-level GPU time with events */
 } ggpu_bfs_opts;
 void ggpu_bfs_opts_default(ggpu_bfs_opts* o);
-/* "gpu_edge" | "gpu_binned" | "gpu_do" | "hybrid" -> GGPU_BFS_* */
+/* "gpu_edge" | "gpu_binned" | "gpu_do" | "hybrid" | "gpu_vertex" | "gpu_naive" -> GGPU_BFS_*
+ * gpu_vertex and gpu_naive are top-down only and run with the host loop and
+ * the heuristic policy (runVertex / runNaive, bfs_gpu.mm:115-175). */
 int ggpu_bfs_parse_variant(const char* name, int* variant);
 
 /* GpuBfs(ctx, g) (bfs_gpu.mm:45-74): copies the symmetric CSR to HBM. */
@@ -120,6 +124,7 @@ int ggpu_msbfs_fetch_dist(ggpu_bfs* b, uint32_t* out);
 #define GGPU_PR_VECTOR 2   /* gpu_pull_vector:   wave per vertex */
 #define GGPU_PR_ADAPTIVE 3 /* gpu_pull_adaptive: vertices binned by in-degree (lane / wave / workgroup) */
 #define GGPU_PR_GS 4       /* gpu_pull_gs:       in-place sweeps in colour or natural order */
+#define GGPU_PR_PUSH 5     /* gpu_push:          float atomics along out-arcs (the transpose of the in-CSR), Jacobi only */
 
 typedef struct ggpu_pr_opts {
     float damping;         /* 0.85 */
@@ -150,7 +155,7 @@ typedef struct ggpu_pr_opts {
  * Jacobi sweeps only: in-place sweeps with this layout use STREAM. */
 #define GGPU_PR_LAYOUT_HOT 2
 void ggpu_pr_opts_default(ggpu_pr_opts* o);
-/* "gpu_pull_scalar" | "gpu_pull_vector" | "gpu_pull_adaptive" | "gpu_pull_gs" -> GGPU_PR_* */
+/* "gpu_pull_scalar" | "gpu_pull_vector" | "gpu_pull_adaptive" | "gpu_pull_gs" | "gpu_push" -> GGPU_PR_* */
 int ggpu_pr_parse_variant(const char* name, int* variant);
 
 typedef struct ggpu_pr_result {

@@ -53,6 +53,26 @@ def test_gpu_edge_and_binned_match_oracle(name):
                 assert sum(L.edges for L in r.trace) == want_edges
 
 
+@pytest.mark.parametrize("name", SUITE)
+def test_gpu_vertex_and_naive_match_oracle(name):
+    """runVertex / runNaive: top-down levels until nothing is discovered; the
+    trace holds one level per depth (the last one discovers nothing), each
+    with the number of vertices at that depth."""
+    g = _sym(name)
+    bfs = G.GpuBfs(g)
+    for src in _sources(g):
+        ref = O.bfs_serial(g.row_ptr, g.col_idx, src)
+        hist = [int(x) for x in np.bincount(ref[ref != O.INF])]
+        want_frontier, want_edges = _reached_stats(g, ref)
+        for variant in ("gpu_vertex", "gpu_naive"):
+            for tg in (64, 256, 1024):
+                r = bfs.run(variant, src, G.bfs_options(tg=tg))
+                assert np.array_equal(r.dist, ref), f"{name} src {src} {variant} tg {tg}"
+                assert [L.frontier for L in r.trace] == hist
+                assert {L.direction for L in r.trace} == {b"T"}
+                assert sum(L.edges for L in r.trace) == want_edges and sum(hist) == want_frontier
+
+
 MODES = [("default", 15.0, 18.0), ("force-bu", 1e9, 1e9), ("never-bu", 1e-9, 18.0), ("flip-flop", 2.0, 2.0)]
 
 
@@ -136,3 +156,8 @@ def test_errors_are_reported():
         bfs.run("gpu_edge", 0, G.bfs_options(ept=0))
     with pytest.raises(ValueError):
         bfs.run("gpu_do", g.n, G.bfs_options())
+    for variant in ("gpu_vertex", "gpu_naive"):
+        with pytest.raises(ValueError):
+            bfs.run(variant, 0, G.bfs_options(loop=G.LOOP_PERSISTENT))
+        with pytest.raises(ValueError):
+            bfs.run(variant, 0, G.bfs_options(policy=G.POLICY_SCRIPTED, actions=[2]))

@@ -38,7 +38,9 @@ CFGS = [("gpu_pull_scalar", {}), ("gpu_pull_vector", {}),
         ("gpu_pull_adaptive", dict(layout=B, small_max=0, large_min=1 << 30)),        # all medium
         ("gpu_pull_adaptive", dict(layout=B, small_max=2, large_min=3)),
         ("gpu_pull_adaptive", dict(layout=B, small_max=4, large_min=64, medium_lanes=8)),
-        ("gpu_pull_adaptive", dict(layout=B, small_max=8, large_min=512, medium_lanes=16))]
+        ("gpu_pull_adaptive", dict(layout=B, small_max=8, large_min=512, medium_lanes=16)),
+        ("gpu_push", {}),                                                   # atomics along out-arcs
+        ("gpu_push", dict(order=G.PR_ORDER_DEGREE))]
 
 
 @pytest.mark.parametrize("name", SUITE)
@@ -69,7 +71,7 @@ def test_pagerank_known_answers_on_gpu():
         g = G.Graph.from_edges(n, edges, directed=True)
         t = g.transpose()
         pr = G.GpuPageRank(t, g.degrees())
-        for v in ("gpu_pull_scalar", "gpu_pull_vector", "gpu_pull_adaptive", "gpu_pull_gs"):
+        for v in ("gpu_pull_scalar", "gpu_pull_vector", "gpu_pull_adaptive", "gpu_pull_gs", "gpu_push"):
             pr.run(v, G.pr_options(iters=200))
             assert np.allclose(pr.ranks(), want, atol=1e-6), v
     for name, n in (("complete200", 200), ("empty1000", 1000)):
@@ -149,6 +151,8 @@ def test_pr_errors_are_reported():
         pr.run("gpu_pull_adaptive", G.pr_options(medium_lanes=3))
     with pytest.raises(ValueError):
         pr.run("gpu_pull_adaptive", G.pr_options(layout=3))
+    with pytest.raises(ValueError):
+        pr.run("gpu_push", G.pr_options(in_place=1))
 
 
 # PPR work assignments: the defaults (heavy in-lists >= 256 entries in chunks,
