@@ -125,6 +125,8 @@ def host_lib() -> C.CDLL:
                                       C.c_uint64, C.POINTER(vp)]
         lib.ggpu_gen_grid.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_uint64, C.POINTER(vp)]
         lib.ggpu_random_perm.argtypes = [C.c_uint64, C.c_uint64, _u32p]
+        lib.ggpu_order_perm.argtypes = [vp, C.c_char_p, C.c_uint64, _u32p]
+        lib.ggpu_graph_locality.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.ggpu_pick_sources.argtypes = [vp, C.c_uint32, C.c_uint64, C.c_int, _u32p, _u32p]
         lib.ggpu_graph_from_arrays.argtypes = [C.c_uint64, C.c_uint64, _u32p, _u32p, C.c_uint32, C.POINTER(vp)]
         lib.ggpu_graph_from_edges.argtypes = [C.c_uint64, C.c_uint64, _u32p, _u32p, C.c_int, C.POINTER(vp)]
@@ -319,6 +321,19 @@ class Graph:
     def permute(self, perm) -> "Graph":
         p = np.ascontiguousarray(perm, dtype=np.uint32)
         return Graph._make(host_lib().ggpu_graph_permute, self._h, _ptr(p))
+
+    def order_perm(self, ordering: str, seed: int = 1) -> np.ndarray:
+        """reorder::compute (src/graph/reorder.cpp:97-103) on this (symmetric)
+        graph: perm[old] = new for "original" | "random" | "degree" | "rcm"."""
+        p = np.zeros(max(self.n, 1), dtype=np.uint32)
+        _host_check(host_lib().ggpu_order_perm(self._h, ordering.encode(), seed, _ptr(p)))
+        return p[: self.n]
+
+    def locality(self):
+        """(mean |u - v|, share of arcs within one 64-vertex block)."""
+        g, s = C.c_double(), C.c_double()
+        _host_check(host_lib().ggpu_graph_locality(self._h, C.byref(g), C.byref(s)))
+        return g.value, s.value
 
     def largest_component(self):
         out = C.c_void_p()

@@ -14,6 +14,8 @@
 //          [--threads N] [--warmup 2] [--trials 5] [--validate] [--csv out.csv] [--dump-ranks out.f64]
 //   gbench --graph X --algo msbfs [--batch 64] [--sources 64] [--policy heuristic|learned]
 //          [--policy-file W.f32] [--warmup 2] [--trials 5] [--validate] [--csv out.csv]
+//   every algorithm: [--ordering original|random|degree|rcm] [--order-seed 1] [--locality-csv loc.csv]
+//          (vertices relabeled before the run, untimed; results mapped back for validation)
 //   gbench --graph X --save-graph out.bin
 //
 // BFS variants: cpu_serial cpu_omp cpu_do (host library) and gpu_vertex
@@ -161,6 +163,12 @@ Graph symmetrized(const Graph& a) {
     return Graph(o);
 }
 
+Graph permuted(const Graph& a, const std::vector<uint32_t>& perm) {
+    ggpu_graph* o = nullptr;
+    host_check(ggpu_graph_permute(a.g, perm.data(), &o), "permute");
+    return Graph(o);
+}
+
 Graph transposed(const Graph& a) {
     ggpu_graph* o = nullptr;
     host_check(ggpu_graph_transpose(a.g, &o), "transpose");
@@ -214,9 +222,40 @@ struct RunInfo {
     double load = load1();
 };
 
-void require_original_ordering(const Args& a) {
-    if (a.get("ordering", "original") != "original")
-        throw Failure("--ordering " + a.get("ordering", "") + " is not built in this backend (original only)");
+// --ordering original|random|degree|rcm (main.cpp:179-201): perm[old] = new
+// computed on the symmetrized graph (empty for "original"); --order-seed for
+// "random"; --locality-csv appends the relabeled graph's locality.
+std::vector<uint32_t> ordering_perm(const Args& a, const Graph& sym, const std::string& dataset) {
+    const std::string ordering = a.get("ordering", "original");
+    std::vector<uint32_t> perm;
+    const double t0 = now_ms();
+    if (ordering != "original") {
+        perm.resize(sym.n());
+        host_check(ggpu_order_perm(sym.g, ordering.c_str(), (uint64_t)a.num("order-seed", 1), perm.data()),
+                   "ordering");
+    }
+    const double order_ms = now_ms() - t0;
+    const std::string loc_csv = a.get("locality-csv", "");
+    if (!loc_csv.empty()) {
+        double gap = 0, same = 0;
+        if (perm.empty()) {
+            host_check(ggpu_graph_locality(sym.g, &gap, &same), "locality");
+        } else {
+            Graph q = permuted(sym, perm);
+            host_check(ggpu_graph_locality(q.g, &gap, &same), "locality");
+        }
+        const bool fresh = !file_exists(loc_csv);
+        std::ofstream f(loc_csv, std::ios::app);
+        if (fresh) f << "key,dataset,ordering,n,m_sym,mean_gap,same_block64,order_ms\n";
+        char row[512];
+        std::snprintf(row, sizeof row, "%s/%s,%s,%s,%llu,%llu,%.2f,%.6f,%.0f\n", dataset.c_str(), ordering.c_str(),
+                      dataset.c_str(), ordering.c_str(), (unsigned long long)sym.n(), (unsigned long long)sym.m(), gap,
+                      same, order_ms);
+        f << row;
+        std::printf("locality (%s): mean |u-v| %.1f, same 64-block %.2f%%, ordering took %.0f ms\n", ordering.c_str(),
+                    gap, 100 * same, order_ms);
+    }
+    return perm;
 }
 
 uint64_t comp_edges(const Graph& g, const std::vector<uint32_t>& dist) {
@@ -268,8 +307,8 @@ int run_bfs(const Args& a, const Graph& stored, const std::string& dataset) {
         std::fprintf(stderr, "\n");
         return 2;
     }
-    require_original_ordering(a);
     const bool gpu = variant != "cpu_serial" && variant != "cpu_omp" && variant != "cpu_do";
+    const std::string ordering = a.get("ordering", "original");
     ggpu_cpu_bfs_opts copt;
     ggpu_cpu_bfs_opts_default(&copt);
     copt.threads = (int)a.num("threads", 0);
@@ -278,7 +317,18 @@ int run_bfs(const Args& a, const Graph& stored, const std::string& dataset) {
     copt.beta = a.real("beta", 18.0);
 
     const double t_prep = now_ms();
+    // ref_g keeps the original labels (sources, the reference BFS, TEPS); g is
+    // what the variant runs on, relabeled when --ordering is not original
     Graph g = symmetrized(stored);
+    const std::vector<uint32_t> perm = ordering_perm(a, g, dataset);
+    const bool relabeled = !perm.empty();
+    Graph ref_store;
+    if (relabeled) {
+        ref_store = std::move(g);
+        g = permuted(ref_store, perm);
+    }
+    const Graph& ref_g = relabeled ? ref_store : g;
+    auto to_run = [&](uint32_t s) { return relabeled ? perm[s] : s; };
     std::unique_ptr<Device> dev;
     ggpu_bfs* gb = nullptr;
     ggpu_bfs_opts gopt;
@@ -313,7 +363,7 @@ int run_bfs(const Args& a, const Graph& stored, const std::string& dataset) {
     } else {
         sources.resize((size_t)std::max<long long>(1, a.num("sources", 64)));
         uint32_t k = 0;
-        host_check(ggpu_pick_sources(g.g, (uint32_t)sources.size(), (uint64_t)a.num("seed", 1),
+        host_check(ggpu_pick_sources(ref_g.g, (uint32_t)sources.size(), (uint64_t)a.num("seed", 1),
                                      a.has("any-component"), sources.data(), &k), "sources");
         sources.resize(k);
     }
@@ -331,7 +381,7 @@ int run_bfs(const Args& a, const Graph& stored, const std::string& dataset) {
     if (variant == "cpu_omp" || variant == "cpu_do")
         params = "threads=" + std::to_string(copt.threads > 0 ? copt.threads : omp_get_max_threads());
     if (variant == "cpu_omp")
-        params += std::string(" schedule=") + (copt.schedule == 0 ? "static" : copt.schedule == 1 ? "dynamic,64" : "guided");
+        params += std::string(" schedule=") + (copt.schedule == 0 ? "static" : copt.schedule == 1 ? "dynamic/64" : "guided");
     char b[160];
     if (variant == "cpu_do") {
         std::snprintf(b, sizeof b, " alpha=%g beta=%g", copt.alpha, copt.beta);
@@ -374,30 +424,34 @@ int run_bfs(const Args& a, const Graph& stored, const std::string& dataset) {
         }
         if (tr) tr->resize(std::min(len, tr->size()));
     };
-    for (int w = 0; w < warmup; ++w) run(sources[w % sources.size()], false, nullptr);
+    for (int w = 0; w < warmup; ++w) run(to_run(sources[w % sources.size()]), false, nullptr);
 
     std::ofstream csv = open_csv(a.get("csv", ""));
     RunInfo info;
     info.chip = no_commas(gpu ? dev->name : cpu_name());
     std::vector<double> all, all_gpu, per_source;
     for (uint32_t s : sources) {
-        host_check(ggpu_cpu_bfs(g.g, "cpu_serial", s, &copt, ref.data(), nullptr, 0, nullptr), "reference");
-        const uint64_t ce = comp_edges(g, ref);
-        for (int w = 0; w < source_warmup && trials > 0; ++w) run(s, false, nullptr);
+        host_check(ggpu_cpu_bfs(ref_g.g, "cpu_serial", s, &copt, ref.data(), nullptr, 0, nullptr), "reference");
+        const uint64_t ce = comp_edges(ref_g, ref);
+        const uint32_t rs = to_run(s);
+        for (int w = 0; w < source_warmup && trials > 0; ++w) run(rs, false, nullptr);
         std::vector<double> times;
         for (int t = 0; t < trials; ++t) {
             const double t0 = now_ms();
-            run(s, false, nullptr);
+            run(rs, false, nullptr);
             const double ms = now_ms() - t0;
             const double gms = gpu ? ggpu_bfs_last_gpu_ms(gb) : -1;
             if (validate) {
                 if (gpu) dev->check(ggpu_bfs_fetch_dist(gb, dist.data()), "fetch");
-                for (uint64_t v = 0; v < g.n(); ++v)
-                    if (dist[v] != ref[v]) {
-                        std::fprintf(stderr, "VALIDATION FAILED: %s [original] source %u trial %d: vertex %llu: got %u, expected %u\n",
-                                     variant.c_str(), s, t, (unsigned long long)v, dist[v], ref[v]);
+                // dist (run ids) equals ref (original ids) after v -> perm[v]
+                for (uint64_t v = 0; v < g.n(); ++v) {
+                    const uint32_t got = dist[relabeled ? perm[v] : v];
+                    if (got != ref[v]) {
+                        std::fprintf(stderr, "VALIDATION FAILED: %s [%s] source %u trial %d: vertex %llu: got %u, expected %u\n",
+                                     variant.c_str(), ordering.c_str(), s, t, (unsigned long long)v, got, ref[v]);
                         return 1;
                     }
+                }
             }
             times.push_back(ms);
             all.push_back(ms);
@@ -406,9 +460,10 @@ int run_bfs(const Args& a, const Graph& stored, const std::string& dataset) {
                 char gf[32] = "";
                 if (gms >= 0) std::snprintf(gf, sizeof gf, "%.4f", gms);
                 char row[1024];
-                std::snprintf(row, sizeof row, "%s,%s,%s,%s,%llu,%llu,original,bfs,%s,%s,%u,%d,%.4f,%s,%.6f,,%d,%.2f\n",
+                std::snprintf(row, sizeof row, "%s,%s,%s,%s,%llu,%llu,%s,bfs,%s,%s,%u,%d,%.4f,%s,%.6f,,%d,%.2f\n",
                               info.ts.c_str(), info.commit.c_str(), info.chip.c_str(), dataset.c_str(),
-                              (unsigned long long)g.n(), (unsigned long long)g.m(), variant.c_str(), params.c_str(), s,
+                              (unsigned long long)g.n(), (unsigned long long)g.m(), ordering.c_str(), variant.c_str(),
+                              params.c_str(), s,
                               t, ms, gf, ce / (ms * 1e6), validate ? 1 : 0, info.load);
                 csv << row;
             }
@@ -435,11 +490,11 @@ int run_bfs(const Args& a, const Graph& stored, const std::string& dataset) {
         std::ofstream tr(trace_path, std::ios::app);
         if (fresh) tr << "dataset,ordering,variant,params,source,level,frontier,edges,direction,device,time_ms\n";
         for (uint32_t s : sources) {
-            run(s, false, &trace);
+            run(to_run(s), false, &trace);
             for (auto& L : trace) {
                 char row[512];
-                std::snprintf(row, sizeof row, "%s,original,%s,%s,%u,%u,%llu,%llu,%c,%c,%.4f\n", dataset.c_str(),
-                              variant.c_str(), params.c_str(), s, L.level, (unsigned long long)L.frontier,
+                std::snprintf(row, sizeof row, "%s,%s,%s,%s,%u,%u,%llu,%llu,%c,%c,%.4f\n", dataset.c_str(),
+                              ordering.c_str(), variant.c_str(), params.c_str(), s, L.level, (unsigned long long)L.frontier,
                               (unsigned long long)L.edges, L.direction, L.device, L.time_ms);
                 tr << row;
             }
@@ -448,7 +503,7 @@ int run_bfs(const Args& a, const Graph& stored, const std::string& dataset) {
     }
     const std::string dump = a.get("dump-dist", "");
     if (!dump.empty()) {
-        run(sources.back(), true, nullptr);
+        run(to_run(sources.back()), true, nullptr);
         std::ofstream out(dump, std::ios::binary);
         out.write(reinterpret_cast<const char*>(dist.data()), g.n() * sizeof(uint32_t));
     }
@@ -457,8 +512,16 @@ int run_bfs(const Args& a, const Graph& stored, const std::string& dataset) {
 
 // ------------------------------------------------------- batched BFS (new)
 int run_msbfs(const Args& a, const Graph& stored, const std::string& dataset) {
-    require_original_ordering(a);
     Graph g = symmetrized(stored);
+    const std::string ordering = a.get("ordering", "original");
+    const std::vector<uint32_t> perm = ordering_perm(a, g, dataset);
+    const bool relabeled = !perm.empty();
+    Graph ref_store;
+    if (relabeled) {
+        ref_store = std::move(g);
+        g = permuted(ref_store, perm);
+    }
+    const Graph& ref_g = relabeled ? ref_store : g;  // original labels
     Device dev;
     ggpu_bfs_opts o;
     ggpu_bfs_opts_default(&o);
@@ -470,9 +533,12 @@ int run_msbfs(const Args& a, const Graph& stored, const std::string& dataset) {
     std::unique_ptr<ggpu_bfs, void (*)(ggpu_bfs*)> owner(gb, ggpu_bfs_destroy);
     std::vector<uint32_t> sources((size_t)std::max<long long>(1, a.num("sources", 64)));
     uint32_t k = 0;
-    host_check(ggpu_pick_sources(g.g, (uint32_t)sources.size(), (uint64_t)a.num("seed", 1), a.has("any-component"),
-                                 sources.data(), &k), "sources");
+    host_check(ggpu_pick_sources(ref_g.g, (uint32_t)sources.size(), (uint64_t)a.num("seed", 1),
+                                 a.has("any-component"), sources.data(), &k), "sources");
     sources.resize(k);
+    std::vector<uint32_t> run_sources(sources);  // run ids
+    if (relabeled)
+        for (auto& x : run_sources) x = perm[x];
     if (sources.empty()) {
         std::fprintf(stderr, "no vertex with degree > 0\n");
         return 1;
@@ -489,8 +555,9 @@ int run_msbfs(const Args& a, const Graph& stored, const std::string& dataset) {
     std::vector<uint32_t> ref(g.n());
     std::vector<uint64_t> ce(sources.size());
     for (size_t i = 0; i < sources.size(); ++i) {
-        host_check(ggpu_cpu_bfs(g.g, "cpu_serial", sources[i], &copt, ref.data(), nullptr, 0, nullptr), "reference");
-        ce[i] = comp_edges(g, ref);
+        host_check(ggpu_cpu_bfs(ref_g.g, "cpu_serial", sources[i], &copt, ref.data(), nullptr, 0, nullptr),
+                   "reference");
+        ce[i] = comp_edges(ref_g, ref);
     }
     std::ofstream csv = open_csv(a.get("csv", ""));
     RunInfo info;
@@ -505,20 +572,23 @@ int run_msbfs(const Args& a, const Graph& stored, const std::string& dataset) {
             const uint32_t kb = (uint32_t)std::min<size_t>(batch, sources.size() - b0);
             ggpu_msbfs_result r{};
             const double t0 = now_ms();
-            dev.check(ggpu_msbfs_run(gb, sources.data() + b0, kb, &o, &r, nullptr, 0, nullptr), "msbfs");
+            dev.check(ggpu_msbfs_run(gb, run_sources.data() + b0, kb, &o, &r, nullptr, 0, nullptr), "msbfs");
             ms += now_ms() - t0;
             gms += r.gpu_ms;
             if (validate && t >= 0) {
                 dist.resize((size_t)kb * g.n());
                 dev.check(ggpu_msbfs_fetch_dist(gb, dist.data()), "fetch");
                 for (uint32_t i = 0; i < kb; ++i) {
-                    host_check(ggpu_cpu_bfs(g.g, "cpu_serial", sources[b0 + i], &copt, ref.data(), nullptr, 0, nullptr),
+                    host_check(ggpu_cpu_bfs(ref_g.g, "cpu_serial", sources[b0 + i], &copt, ref.data(), nullptr, 0,
+                                            nullptr),
                                "reference");
-                    if (std::memcmp(ref.data(), dist.data() + (size_t)i * g.n(), g.n() * 4) != 0) {
-                        std::fprintf(stderr, "VALIDATION FAILED: msbfs source %u (batch position %u) trial %d\n",
-                                     sources[b0 + i], i, t);
-                        return 1;
-                    }
+                    const uint32_t* d = dist.data() + (size_t)i * g.n();
+                    for (uint64_t v = 0; v < g.n(); ++v)
+                        if (d[relabeled ? perm[v] : v] != ref[v]) {
+                            std::fprintf(stderr, "VALIDATION FAILED: msbfs [%s] source %u (batch position %u) trial %d\n",
+                                         ordering.c_str(), sources[b0 + i], i, t);
+                            return 1;
+                        }
                 }
             }
         }
@@ -527,9 +597,10 @@ int run_msbfs(const Args& a, const Graph& stored, const std::string& dataset) {
         all_gpu.push_back(gms);
         if (csv.is_open()) {
             char row[1024];
-            std::snprintf(row, sizeof row, "%s,%s,%s,%s,%llu,%llu,original,msbfs,gpu_msbfs,%s,%u,%d,%.4f,%.4f,%.6f,,%d,%.2f\n",
+            std::snprintf(row, sizeof row, "%s,%s,%s,%s,%llu,%llu,%s,msbfs,gpu_msbfs,%s,%u,%d,%.4f,%.4f,%.6f,,%d,%.2f\n",
                           info.ts.c_str(), info.commit.c_str(), info.chip.c_str(), dataset.c_str(),
-                          (unsigned long long)g.n(), (unsigned long long)g.m(), params.c_str(), sources[0], t, ms, gms,
+                          (unsigned long long)g.n(), (unsigned long long)g.m(), ordering.c_str(), params.c_str(),
+                          sources[0], t, ms, gms,
                           total_edges / (ms * 1e6), validate ? 1 : 0, info.load);
             csv << row;
         }
@@ -570,8 +641,8 @@ int run_pr(const Args& a, const Graph& stored, const std::string& dataset) {
         std::fprintf(stderr, "\n");
         return 2;
     }
-    require_original_ordering(a);
     const bool gpu = variant.rfind("gpu_", 0) == 0;
+    const std::string ordering = a.get("ordering", "original");
     const double tol = a.real("tol", 0);
     const int iters = (int)a.num("iters", tol > 0 ? 1000 : 20);
     const int threads = (int)a.num("threads", 0);
@@ -608,6 +679,28 @@ int run_pr(const Args& a, const Graph& stored, const std::string& dataset) {
             ggpu_check_ranks(n, ref_conv.data(), rf.data(), &jacobi.l1, &jacobi.mx, &jacobi.sum);
         }
     }
+    // --ordering: perm from the symmetrized graph (as for BFS) relabels the
+    // directed graph the variant runs on; the reference above keeps the
+    // original labels and ranks are mapped back (rank of v = run[perm[v]])
+    std::vector<uint32_t> perm;
+    {
+        Graph sym = symmetrized(stored);
+        perm = ordering_perm(a, sym, dataset);
+    }
+    const bool relabeled = !perm.empty();
+    Graph run_graph, run_in_owned;
+    const Graph* run_in = in;
+    std::vector<uint32_t> run_outdeg = outdeg;
+    if (relabeled) {
+        run_graph = permuted(stored, perm);
+        if (run_graph.directed()) {
+            run_in_owned = transposed(run_graph);
+            run_in = &run_in_owned;
+        } else {
+            run_in = &run_graph;
+        }
+        for (uint64_t v = 0; v < n; ++v) run_outdeg[perm[v]] = outdeg[v];
+    }
     std::unique_ptr<Device> dev;
     ggpu_pr* pr = nullptr;
     ggpu_pr_opts po;
@@ -623,8 +716,8 @@ int run_pr(const Args& a, const Graph& stored, const std::string& dataset) {
         po.large_min = (uint32_t)a.num("large-min", 256);
         po.in_place = in_place;
         dev->check(ggpu_pr_parse_variant(variant.c_str(), &pvariant), "variant");
-        dev->check(ggpu_pr_create(dev->ctx, n, in->m(), ggpu_graph_row_ptr(in->g), ggpu_graph_col_idx(in->g),
-                                  outdeg.data(), &pr), "upload");
+        dev->check(ggpu_pr_create(dev->ctx, n, run_in->m(), ggpu_graph_row_ptr(run_in->g),
+                                  ggpu_graph_col_idx(run_in->g), run_outdeg.data(), &pr), "upload");
         if (variant == "gpu_pull_adaptive") dev->check(ggpu_pr_prepare_adaptive(pr, po.small_max, po.large_min), "prepare");
         if (variant == "gpu_pull_gs") dev->check(ggpu_pr_prepare_gs(pr, (uint32_t)a.num("gs-pairs", 2)), "prepare");
     }
@@ -651,9 +744,9 @@ int run_pr(const Args& a, const Graph& stored, const std::string& dataset) {
     }
     if (variant == "gpu_pull_adaptive")
         params += " small_max=" + std::to_string(po.small_max) + " large_min=" + std::to_string(po.large_min);
-    std::printf("%s: n %llu, m %llu (directed=%d), ordering original, prep %.0f ms\npr %s [%s]\n", dataset.c_str(),
-                (unsigned long long)n, (unsigned long long)m, (int)stored.directed(), prep_ms, variant.c_str(),
-                params.c_str());
+    std::printf("%s: n %llu, m %llu (directed=%d), ordering %s, prep %.0f ms\npr %s [%s]\n", dataset.c_str(),
+                (unsigned long long)n, (unsigned long long)m, (int)stored.directed(), ordering.c_str(), prep_ms,
+                variant.c_str(), params.c_str());
 
     std::vector<double> rcpu(n);
     std::vector<float> ranks(n);
@@ -662,7 +755,7 @@ int run_pr(const Args& a, const Graph& stored, const std::string& dataset) {
     auto run_once = [&]() -> std::pair<int, double> {
         if (!gpu) {
             int it = 0;
-            host_check(ggpu_cpu_pagerank(in->g, outdeg.data(), 0.85, iters, tol, variant == "cpu_omp", threads,
+            host_check(ggpu_cpu_pagerank(run_in->g, run_outdeg.data(), 0.85, iters, tol, variant == "cpu_omp", threads,
                                          rcpu.data(), &it, &last_delta), "pagerank");
             return {it, -1.0};
         }
@@ -673,10 +766,12 @@ int run_pr(const Args& a, const Graph& stored, const std::string& dataset) {
         floor_stop = r.floor_stop;
         return {r.iters, r.gpu_ms};
     };
-    auto current = [&]() {
-        if (gpu) dev->check(ggpu_pr_fetch_ranks(pr, ranks.data()), "fetch");
+    std::vector<float> run_ranks(n);
+    auto current = [&]() {  // ranks in the original labels
+        if (gpu) dev->check(ggpu_pr_fetch_ranks(pr, run_ranks.data()), "fetch");
         else
-            for (uint64_t v = 0; v < n; ++v) ranks[v] = (float)rcpu[v];
+            for (uint64_t v = 0; v < n; ++v) run_ranks[v] = (float)rcpu[v];
+        for (uint64_t v = 0; v < n; ++v) ranks[v] = run_ranks[relabeled ? perm[v] : v];
         return ranks;
     };
     const int warmup = (int)a.num("warmup", 2), trials = (int)a.num("trials", 5);
@@ -709,9 +804,10 @@ int run_pr(const Args& a, const Graph& stored, const std::string& dataset) {
             char gf[32] = "";
             if (gms >= 0) std::snprintf(gf, sizeof gf, "%.4f", gms);
             char row[1024];
-            std::snprintf(row, sizeof row, "%s,%s,%s,%s,%llu,%llu,original,pr,%s,%s,,%d,%.4f,%s,%.6f,%d,%d,%.2f\n",
+            std::snprintf(row, sizeof row, "%s,%s,%s,%s,%llu,%llu,%s,pr,%s,%s,,%d,%.4f,%s,%.6f,%d,%d,%.2f\n",
                           info.ts.c_str(), info.commit.c_str(), info.chip.c_str(), dataset.c_str(),
-                          (unsigned long long)n, (unsigned long long)m, variant.c_str(), params.c_str(), t, ms, gf,
+                          (unsigned long long)n, (unsigned long long)m, ordering.c_str(), variant.c_str(),
+                          params.c_str(), t, ms, gf,
                           (double)m * it / (ms * 1e6), it, validate ? 1 : 0, info.load);
             csv << row;
         }

@@ -754,7 +754,18 @@ void BfsEngine::device_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, s
         GGPU_HIP_TRY(hipMemcpyAsync(&abort, D.abort, 4, hipMemcpyDeviceToHost, s));
         GGPU_HIP_TRY(hipMemcpyAsync(&out, R.ds.p, sizeof out, hipMemcpyDeviceToHost, s));
         GGPU_HIP_TRY(hipStreamSynchronize(s));
-        if (abort) throw ArgError("persistent BFS: grid barrier timed out", GGPU_ETIMEOUT);
+        if (abort) {
+            // where the grid stood: arrivals per barrier group, top counter,
+            // generation, single-workgroup episodes handed back
+            std::vector<uint32_t> c(kBarWords + 8);
+            GGPU_HIP_TRY(hipMemcpy(c.data(), R.ctl.p, c.size() * 4, hipMemcpyDeviceToHost));
+            std::string msg = "persistent BFS: grid barrier timed out (grid " + std::to_string(R.persist_grid) +
+                              ", group arrivals";
+            for (uint32_t g = 0; g < kBarGroups; ++g) msg += " " + std::to_string(c[g * kBarStride]);
+            msg += ", top " + std::to_string(c[kBarTop]) + ", generation " + std::to_string(c[kBarGen]) +
+                   ", episodes " + std::to_string(c[kBarWords + 1]) + ", level " + std::to_string(out.st.level) + ")";
+            throw ArgError(msg, GGPU_ETIMEOUT);
+        }
     } else {
         // a graph of K levels; the captured kernels read everything else from device memory
         if (!R.exec || R.graph_k != o.graph_levels || std::memcmp(&R.graph_dev, &D, sizeof D) != 0) {

@@ -68,6 +68,13 @@ Csr transpose(const Csr& g);
 Csr symmetrize(const Csr& g);
 Csr permute(const Csr& g, const std::vector<uint32_t>& perm);
 std::vector<uint32_t> random_perm(uint64_t n, uint64_t seed);
+// reorder.cpp: --ordering original|random|degree|rcm on a symmetric graph
+std::vector<uint32_t> order_perm(const Csr& sym, const std::string& ordering, uint64_t seed);
+struct Locality {
+    double mean_gap = 0;      // mean |u - v| over the stored arcs
+    double same_block64 = 0;  // share of arcs with u / 64 == v / 64
+};
+Locality locality(const Csr& g);
 std::vector<uint32_t> out_degrees(const Csr& g);
 // Component id per vertex (BFS labeling in increasing root order) and the
 // id of the largest component (first one found among equal sizes).

@@ -129,6 +129,23 @@ int ggpu_random_perm(uint64_t n, uint64_t seed, uint32_t* perm_out) {
     });
 }
 
+int ggpu_order_perm(const ggpu_graph* sym, const char* ordering, uint64_t seed, uint32_t* perm_out) {
+    if (!sym || !ordering || (!perm_out && sym->g.n)) return fail(GGPU_EINVAL, "null argument");
+    return guarded([&] {
+        std::vector<uint32_t> p = ggpu::order_perm(sym->g, ordering, seed);
+        if (!p.empty()) std::memcpy(perm_out, p.data(), 4 * p.size());
+    });
+}
+
+int ggpu_graph_locality(const ggpu_graph* g, double* mean_gap, double* same_block64) {
+    if (!g || !mean_gap || !same_block64) return fail(GGPU_EINVAL, "null argument");
+    return guarded([&] {
+        const ggpu::Locality l = ggpu::locality(g->g);
+        *mean_gap = l.mean_gap;
+        *same_block64 = l.same_block64;
+    });
+}
+
 int ggpu_graph_largest_component(const ggpu_graph* g, ggpu_graph** out, uint32_t* old_ids_out, uint64_t* n_out) {
     if (!g || !out) return fail(GGPU_EINVAL, "null argument");
     return guarded([&] {

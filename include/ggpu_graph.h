@@ -51,6 +51,14 @@ int ggpu_graph_symmetrize(const ggpu_graph* g, ggpu_graph** out);
 int ggpu_graph_permute(const ggpu_graph* g, const uint32_t* perm, ggpu_graph** out);
 /* Fisher-Yates permutation with the SplitMix64 generator (reorder.cpp:20-25). */
 int ggpu_random_perm(uint64_t n, uint64_t seed, uint32_t* perm_out);
+/* graph::reorder::compute(sym, ordering, seed) (src/graph/reorder.cpp:97-103):
+ * "original" | "random" | "degree" (descending degree, ties by id) | "rcm"
+ * (reverse Cuthill-McKee per component from a pseudo-peripheral start).
+ * perm_out[old] = new, n entries; computed on a symmetric graph. */
+int ggpu_order_perm(const ggpu_graph* sym, const char* ordering, uint64_t seed, uint32_t* perm_out);
+/* graph::reorder::locality (reorder.cpp:113-125): mean |u - v| over the
+ * stored arcs and the share of arcs within one 64-vertex block. */
+int ggpu_graph_locality(const ggpu_graph* g, double* mean_gap, double* same_block64);
 /* Largest connected component of a symmetric graph, relabeled in increasing
  * original-id order (row-major for grids); old_ids_out (optional, n entries)
  * receives the original id of each new vertex. */

@@ -73,6 +73,10 @@ def ref_lib() -> Optional[C.CDLL]:
         L.ref_symmetrize.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, u32p, u32p, u64p, u64p, u32p, pp, pp]
         L.ref_transpose.argtypes = L.ref_symmetrize.argtypes
         L.ref_random_perm.argtypes = [C.c_uint64, C.c_uint64, u32p]
+        L.ref_order_perm.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, u32p, u32p, C.c_char_p, C.c_uint64, u32p]
+        L.ref_locality.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, u32p, u32p, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+        L.ref_locality.restype = None
         L.ref_permute.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, u32p, u32p, u32p, u64p, u64p, u32p, pp, pp]
         L.ref_bfs_serial.argtypes = [C.c_uint64, C.c_uint64, u32p, u32p, C.c_uint32, u32p]
         L.ref_bfs_serial.restype = None
@@ -241,6 +245,25 @@ def ref_random_perm(n, seed):
     p = np.zeros(max(n, 1), np.uint32)
     ref_lib().ref_random_perm(n, seed, _p(p))
     return p[:n]
+
+
+def ref_order_perm(rp, ci, flags, ordering, seed=1):
+    """graph::reorder::compute of the compiled reference."""
+    rp = np.ascontiguousarray(rp, np.uint32)
+    ci = np.ascontiguousarray(ci, np.uint32)
+    n = len(rp) - 1
+    p = np.zeros(max(n, 1), np.uint32)
+    if ref_lib().ref_order_perm(n, len(ci), flags, _p(rp), _p(ci), ordering.encode(), seed, _p(p)):
+        raise ValueError(ordering)
+    return p[:n]
+
+
+def ref_locality(rp, ci, flags):
+    rp = np.ascontiguousarray(rp, np.uint32)
+    ci = np.ascontiguousarray(ci, np.uint32)
+    g, s = C.c_double(), C.c_double()
+    ref_lib().ref_locality(len(rp) - 1, len(ci), flags, _p(rp), _p(ci), C.byref(g), C.byref(s))
+    return g.value, s.value
 
 
 def ref_bfs_serial(rp, ci, src):

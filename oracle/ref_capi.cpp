@@ -90,6 +90,24 @@ int ref_random_perm(uint64_t n, uint64_t seed, uint32_t* perm) {
     return 0;
 }
 
+int ref_order_perm(uint64_t n, uint64_t m, uint32_t fl, const uint32_t* rp, const uint32_t* ci, const char* ordering,
+                   uint64_t seed, uint32_t* perm) {
+    try {
+        std::vector<uint32_t> p = graph::reorder::compute(view(n, m, fl, rp, ci), ordering, seed);
+        if (n) std::memcpy(perm, p.data(), 4 * n);
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
+
+void ref_locality(uint64_t n, uint64_t m, uint32_t fl, const uint32_t* rp, const uint32_t* ci, double* gap,
+                  double* same) {
+    const graph::reorder::Locality l = graph::reorder::locality(view(n, m, fl, rp, ci));
+    *gap = l.meanGap;
+    *same = l.sameBlock64;
+}
+
 int ref_permute(uint64_t n, uint64_t m, uint32_t fl, const uint32_t* rp, const uint32_t* ci, const uint32_t* perm,
                 uint64_t* on, uint64_t* om, uint32_t* oflags, uint32_t** orp, uint32_t** oci) {
     std::vector<uint32_t> p(perm, perm + n);

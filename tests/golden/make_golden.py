@@ -47,7 +47,7 @@ def out_degrees(rp):
 def main():
     if O.ref_lib() is None:
         sys.exit("oracle/_ref/libggpu_ref.so is not built (make -C oracle)")
-    fx = {"suite": {}, "rmat": [], "random_perm": []}
+    fx = {"suite": {}, "rmat": [], "random_perm": [], "orderings": []}
     for name in SUITE:
         rp, ci, fl = O.ref_suite(name)
         n = len(rp) - 1
@@ -78,6 +78,14 @@ def main():
     for n, seed in [(10, 1), (1000, 2), (65536, 5)]:
         p = O.ref_random_perm(n, seed)
         fx["random_perm"].append({"n": n, "seed": seed, "sha1": sha(p, "<u4"), "head": p[:8].tolist()})
+    for name in ("two_components", "grid100x100", "rmat12", "star100k"):
+        rp, ci, fl = O.ref_symmetrize(*O.ref_suite(name))
+        for ordering, seed in (("random", 3), ("degree", 1), ("rcm", 1)):
+            p = O.ref_order_perm(rp, ci, fl, ordering, seed)
+            qrp, qci, qfl = O.ref_permute(rp, ci, fl, p)
+            gap, same = O.ref_locality(qrp, qci, qfl)
+            fx["orderings"].append({"graph": name, "ordering": ordering, "seed": seed, "sha1": sha(p, "<u4"),
+                                    "head": p[:8].tolist(), "mean_gap": gap, "same_block64": same})
     path = os.path.join(HERE, "ref_vectors.json")
     with open(path, "w") as f:
         json.dump(fx, f, indent=0, sort_keys=True)
