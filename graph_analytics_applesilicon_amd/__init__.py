@@ -126,6 +126,9 @@ def host_lib() -> C.CDLL:
         lib.ggpu_gen_grid.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_uint64, C.POINTER(vp)]
         lib.ggpu_random_perm.argtypes = [C.c_uint64, C.c_uint64, _u32p]
         lib.ggpu_order_perm.argtypes = [vp, C.c_char_p, C.c_uint64, _u32p]
+        lib.ggpu_graph_from_snap.argtypes = [C.c_char_p, C.c_int, C.POINTER(vp), C.POINTER(SnapStats)]
+        lib.ggpu_graph_stats_compute.argtypes = [vp, C.POINTER(GraphStats)]
+        lib.ggpu_degree_histogram.argtypes = [vp, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
         lib.ggpu_graph_locality.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.ggpu_pick_sources.argtypes = [vp, C.c_uint32, C.c_uint64, C.c_int, _u32p, _u32p]
         lib.ggpu_graph_from_arrays.argtypes = [C.c_uint64, C.c_uint64, _u32p, _u32p, C.c_uint32, C.POINTER(vp)]
@@ -260,6 +263,29 @@ class Graph:
         return Graph._make(host_lib().ggpu_graph_load, path.encode())
 
     @staticmethod
+    def from_snap(path: str, directed: bool):
+        """graph::convertSnap (src/graph/snap.cpp:85-148) without the file:
+        (Graph, SnapStats) from a plain or gzip SNAP edge list."""
+        out = C.c_void_p()
+        st = SnapStats()
+        _host_check(host_lib().ggpu_graph_from_snap(str(path).encode(), int(directed), C.byref(out), C.byref(st)))
+        return Graph(out), st
+
+    def stats(self) -> "GraphStats":
+        """The figures gstats reports (src/graph/stats.cpp:40-120)."""
+        st = GraphStats()
+        _host_check(host_lib().ggpu_graph_stats_compute(self._h, C.byref(st)))
+        return st
+
+    def degree_histogram(self, kind: int = 0) -> np.ndarray:
+        """log2 buckets (0: degree 0, k: [2^(k-1), 2^k)) of stored (0), in (1)
+        or symmetrized (2) degrees."""
+        h = (C.c_uint64 * 34)()
+        k = C.c_uint32()
+        _host_check(host_lib().ggpu_degree_histogram(self._h, kind, h, C.byref(k)))
+        return np.array(h[: k.value], dtype=np.uint64)
+
+    @staticmethod
     def from_arrays(row_ptr, col_idx, flags=FLAG_SORTED | FLAG_DEDUP) -> "Graph":
         rp = np.ascontiguousarray(row_ptr, dtype=np.uint32)
         ci = np.ascontiguousarray(col_idx, dtype=np.uint32)
@@ -378,6 +404,18 @@ class Graph:
         _host_check(host_lib().ggpu_cpu_pagerank(self._h, _ptr(od), damping, iters, tol, int(parallel), threads,
                                                  _ptr(r, C.c_double), C.byref(it), C.byref(dl)))
         return r[: self.n], it.value, dl.value
+
+
+class SnapStats(C.Structure):
+    _fields_ = [("lines", C.c_uint64), ("raw_max_id", C.c_uint64), ("n", C.c_uint64), ("m_before", C.c_uint64),
+                ("m", C.c_uint64)]
+
+
+class GraphStats(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("m", C.c_uint64), ("undirected_edges", C.c_uint64), ("min_deg", C.c_uint32),
+                ("max_deg", C.c_uint32), ("max_in_deg", C.c_uint32), ("approx_diameter", C.c_uint32),
+                ("mean_deg", C.c_double), ("isolated", C.c_uint64), ("num_cc", C.c_uint64),
+                ("largest_cc", C.c_uint64)]
 
 
 def random_perm(n: int, seed: int) -> np.ndarray:

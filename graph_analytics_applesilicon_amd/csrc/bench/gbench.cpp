@@ -42,6 +42,7 @@
 #include <string>
 #include <vector>
 
+#include "cli.hpp"
 #include "ggpu_graph.h"
 #include "ggpu_hip.h"
 
@@ -224,7 +225,8 @@ struct RunInfo {
 
 // --ordering original|random|degree|rcm (main.cpp:179-201): perm[old] = new
 // computed on the symmetrized graph (empty for "original"); --order-seed for
-// "random"; --locality-csv appends the relabeled graph's locality.
+// "random"; --locality-csv keeps one row per dataset/ordering with the
+// relabeled graph's locality.
 std::vector<uint32_t> ordering_perm(const Args& a, const Graph& sym, const std::string& dataset) {
     const std::string ordering = a.get("ordering", "original");
     std::vector<uint32_t> perm;
@@ -244,14 +246,12 @@ std::vector<uint32_t> ordering_perm(const Args& a, const Graph& sym, const std::
             Graph q = permuted(sym, perm);
             host_check(ggpu_graph_locality(q.g, &gap, &same), "locality");
         }
-        const bool fresh = !file_exists(loc_csv);
-        std::ofstream f(loc_csv, std::ios::app);
-        if (fresh) f << "key,dataset,ordering,n,m_sym,mean_gap,same_block64,order_ms\n";
         char row[512];
-        std::snprintf(row, sizeof row, "%s/%s,%s,%s,%llu,%llu,%.2f,%.6f,%.0f\n", dataset.c_str(), ordering.c_str(),
+        std::snprintf(row, sizeof row, "%s/%s,%s,%s,%llu,%llu,%.2f,%.6f,%.0f", dataset.c_str(), ordering.c_str(),
                       dataset.c_str(), ordering.c_str(), (unsigned long long)sym.n(), (unsigned long long)sym.m(), gap,
                       same, order_ms);
-        f << row;
+        cli::csv_upsert(loc_csv, "key,dataset,ordering,n,m_sym,mean_gap,same_block64,order_ms",
+                        dataset + "/" + ordering, {row});
         std::printf("locality (%s): mean |u-v| %.1f, same 64-block %.2f%%, ordering took %.0f ms\n", ordering.c_str(),
                     gap, 100 * same, order_ms);
     }

@@ -70,6 +70,8 @@ void canonicalize(uint64_t n, std::vector<uint32_t>& rp, std::vector<uint32_t>& 
 
 }  // namespace
 
+void canonicalize_lists(uint64_t n, std::vector<uint32_t>& rp, std::vector<uint32_t>& ci) { canonicalize(n, rp, ci); }
+
 void save_bin(const Csr& g, const std::string& path) {
     FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) throw std::runtime_error("cannot write " + path);

@@ -75,6 +75,23 @@ struct Locality {
     double same_block64 = 0;  // share of arcs with u / 64 == v / 64
 };
 Locality locality(const Csr& g);
+// sorts each list, drops duplicates and self-loops, packs col_idx
+void canonicalize_lists(uint64_t n, std::vector<uint32_t>& rp, std::vector<uint32_t>& ci);
+
+// ---- snap.cpp: edge-list ingest (snap.cpp:85-148) and gstats figures (stats.cpp:40-120)
+struct SnapStats {
+    uint64_t lines = 0, raw_max_id = 0, n = 0, m_before = 0, m = 0;
+};
+Csr load_snap(const std::string& path, bool directed, SnapStats* st);
+struct GraphStats {
+    uint64_t n = 0, m = 0, undirected_edges = 0;
+    uint32_t min_deg = 0, max_deg = 0, max_in_deg = 0;
+    double mean_deg = 0;
+    uint64_t isolated = 0, num_cc = 0, largest_cc = 0;
+    uint32_t approx_diameter = 0;
+};
+GraphStats graph_stats(const Csr& g);
+std::vector<uint64_t> log2_histogram(const std::vector<uint32_t>& deg);
 std::vector<uint32_t> out_degrees(const Csr& g);
 // Component id per vertex (BFS labeling in increasing root order) and the
 // id of the largest component (first one found among equal sizes).

@@ -137,6 +137,57 @@ int ggpu_order_perm(const ggpu_graph* sym, const char* ordering, uint64_t seed, 
     });
 }
 
+int ggpu_graph_from_snap(const char* path, int directed, ggpu_graph** out, ggpu_snap_stats* stats) {
+    if (!path || !out) return fail(GGPU_EINVAL, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        ggpu::SnapStats st;
+        auto* g = new ggpu_graph{ggpu::load_snap(path, directed != 0, &st)};
+        *out = g;
+        if (stats) *stats = ggpu_snap_stats{st.lines, st.raw_max_id, st.n, st.m_before, st.m};
+    });
+}
+
+int ggpu_graph_stats_compute(const ggpu_graph* g, ggpu_graph_stats* out) {
+    if (!g || !out) return fail(GGPU_EINVAL, "null argument");
+    return guarded([&] {
+        const ggpu::GraphStats t = ggpu::graph_stats(g->g);
+        ggpu_graph_stats o{};
+        o.n = t.n;
+        o.m = t.m;
+        o.undirected_edges = t.undirected_edges;
+        o.min_deg = t.min_deg;
+        o.max_deg = t.max_deg;
+        o.max_in_deg = t.max_in_deg;
+        o.approx_diameter = t.approx_diameter;
+        o.mean_deg = t.mean_deg;
+        o.isolated = t.isolated;
+        o.num_cc = t.num_cc;
+        o.largest_cc = t.largest_cc;
+        *out = o;
+    });
+}
+
+int ggpu_degree_histogram(const ggpu_graph* g, int kind, uint64_t* hist, uint32_t* buckets) {
+    if (!g || !hist || !buckets) return fail(GGPU_EINVAL, "null argument");
+    if (kind < 0 || kind > 2) return fail(GGPU_EINVAL, "kind must be 0 (stored), 1 (in) or 2 (symmetrized)");
+    return guarded([&] {
+        const ggpu::Csr& c = g->g;
+        std::vector<uint32_t> deg(c.n, 0);
+        if (kind == 0) {
+            for (uint64_t v = 0; v < c.n; ++v) deg[v] = c.deg(v);
+        } else if (kind == 1) {
+            for (uint64_t e = 0; e < c.m; ++e) ++deg[c.ci[e]];
+        } else {
+            const ggpu::Csr s = ggpu::symmetrize(c);
+            for (uint64_t v = 0; v < c.n; ++v) deg[v] = s.deg(v);
+        }
+        const std::vector<uint64_t> h = ggpu::log2_histogram(deg);
+        std::memcpy(hist, h.data(), h.size() * sizeof(uint64_t));
+        *buckets = (uint32_t)h.size();
+    });
+}
+
 int ggpu_graph_locality(const ggpu_graph* g, double* mean_gap, double* same_block64) {
     if (!g || !mean_gap || !same_block64) return fail(GGPU_EINVAL, "null argument");
     return guarded([&] {

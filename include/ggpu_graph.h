@@ -59,6 +59,36 @@ int ggpu_order_perm(const ggpu_graph* sym, const char* ordering, uint64_t seed, 
 /* graph::reorder::locality (reorder.cpp:113-125): mean |u - v| over the
  * stored arcs and the share of arcs within one 64-vertex block. */
 int ggpu_graph_locality(const ggpu_graph* g, double* mean_gap, double* same_block64);
+
+/* graph::convertSnap (src/graph/snap.cpp:85-148) without the file write: a
+ * SNAP edge list (plain or gzip text, "u v" lines, '#'/'%' comments, extra
+ * columns ignored) as a CSR; the raw IDs that occur are numbered densely in
+ * increasing order; directed = 0 stores both directions of each line; lists
+ * sorted, deduplicated, self-loops dropped. Save with ggpu_graph_save. */
+typedef struct ggpu_snap_stats {
+    uint64_t lines;      /* edge lines parsed (comments excluded) */
+    uint64_t raw_max_id; /* largest raw vertex ID */
+    uint64_t n;          /* vertices after relabeling */
+    uint64_t m_before;   /* list entries before dedup / self-loop removal */
+    uint64_t m;          /* list entries kept */
+} ggpu_snap_stats;
+int ggpu_graph_from_snap(const char* path, int directed, ggpu_graph** out, ggpu_snap_stats* stats);
+
+/* The figures gstats prints (src/graph/stats.cpp:40-120): stored-graph
+ * degrees (out-degrees when directed), isolated vertices, connected
+ * components and an approximate diameter (deepest BFS level from 10
+ * SplitMix64(1) sources of nonzero degree) of the symmetrized graph. */
+typedef struct ggpu_graph_stats {
+    uint64_t n, m, undirected_edges;
+    uint32_t min_deg, max_deg, max_in_deg, approx_diameter;
+    double mean_deg;
+    uint64_t isolated, num_cc, largest_cc;
+} ggpu_graph_stats;
+int ggpu_graph_stats_compute(const ggpu_graph* g, ggpu_graph_stats* out);
+/* log2 degree histogram: bucket 0 = degree 0, bucket k = [2^(k-1), 2^k);
+ * kind 0 stored (out-) degrees, 1 in-degrees, 2 symmetrized degrees.
+ * hist has room for 34 buckets; *buckets = used (trailing zeros dropped). */
+int ggpu_degree_histogram(const ggpu_graph* g, int kind, uint64_t* hist, uint32_t* buckets);
 /* Largest connected component of a symmetric graph, relabeled in increasing
  * original-id order (row-major for grids); old_ids_out (optional, n entries)
  * receives the original id of each new vertex. */

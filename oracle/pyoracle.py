@@ -74,6 +74,8 @@ def ref_lib() -> Optional[C.CDLL]:
         L.ref_transpose.argtypes = L.ref_symmetrize.argtypes
         L.ref_random_perm.argtypes = [C.c_uint64, C.c_uint64, u32p]
         L.ref_order_perm.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, u32p, u32p, C.c_char_p, C.c_uint64, u32p]
+        L.ref_convert_snap.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_uint64), C.c_char_p,
+                                       C.c_size_t]
         L.ref_locality.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, u32p, u32p, C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
         L.ref_locality.restype = None
@@ -256,6 +258,16 @@ def ref_order_perm(rp, ci, flags, ordering, seed=1):
     if ref_lib().ref_order_perm(n, len(ci), flags, _p(rp), _p(ci), ordering.encode(), seed, _p(p)):
         raise ValueError(ordering)
     return p[:n]
+
+
+def ref_convert_snap(src, dst, directed):
+    """graph::convertSnap of the compiled reference: writes dst (.bin); returns
+    (lines, raw_max_id, n, m_before, m), or raises ValueError with its message."""
+    st = (C.c_uint64 * 5)()
+    err = C.create_string_buffer(512)
+    if ref_lib().ref_convert_snap(str(src).encode(), str(dst).encode(), int(directed), st, err, 512):
+        raise ValueError(err.value.decode())
+    return tuple(int(x) for x in st)
 
 
 def ref_locality(rp, ci, flags):

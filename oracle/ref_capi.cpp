@@ -2,10 +2,11 @@
 //
 // TEST INFRASTRUCTURE ONLY. oracle/Makefile compiles this file together with
 // the reference's sources where they lie (src/graph/{csr,transform,generators,
-// reorder}.cpp and src/cpu/{bfs_cpu,bfs_steps,pagerank_cpu}.cpp under the
+// reorder,snap}.cpp and src/cpu/{bfs_cpu,bfs_steps,pagerank_cpu}.cpp under the
 // reference checkout) into oracle/_ref/libggpu_ref.so. Nothing from the
 // reference is copied here: this file only calls its public functions
 // (graph::gen::*, graph::transpose/symmetrize/permute, graph::reorder::compute,
+// graph::convertSnap,
 // cpu::bfsSerial/bfsDO/bfsOmp, cpu::pagerankSerial/pagerankOmp) so tests can
 // pin the oracle restatement against the real thing, and bench.py can time
 // the reference's cpu_do / cpu_omp as the CPU baseline.
@@ -20,6 +21,7 @@
 #include "graph/csr.hpp"
 #include "graph/generators.hpp"
 #include "graph/reorder.hpp"
+#include "graph/snap.hpp"
 #include "graph/transform.hpp"
 
 namespace {
@@ -106,6 +108,25 @@ void ref_locality(uint64_t n, uint64_t m, uint32_t fl, const uint32_t* rp, const
     const graph::reorder::Locality l = graph::reorder::locality(view(n, m, fl, rp, ci));
     *gap = l.meanGap;
     *same = l.sameBlock64;
+}
+
+// graph::convertSnap: edge list -> .bin file; stats[5] = lines, raw max id,
+// n, entries before and after compaction. Returns 0, or 1 with the message in
+// err (err_len bytes) on an error the converter reports.
+int ref_convert_snap(const char* in, const char* out, int directed, uint64_t* stats, char* err, size_t err_len) {
+    try {
+        const graph::ConvertStats st = graph::convertSnap(in, out, directed != 0);
+        stats[0] = st.lines;
+        stats[1] = st.raw_max_id;
+        stats[2] = st.n;
+        stats[3] = st.m_before;
+        stats[4] = st.m;
+        return 0;
+    } catch (const std::exception& e) {
+        std::strncpy(err, e.what(), err_len - 1);
+        err[err_len - 1] = 0;
+        return 1;
+    }
 }
 
 int ref_permute(uint64_t n, uint64_t m, uint32_t fl, const uint32_t* rp, const uint32_t* ci, const uint32_t* perm,
