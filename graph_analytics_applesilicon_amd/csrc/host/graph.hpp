@@ -91,6 +91,8 @@ struct GraphStats {
     uint32_t approx_diameter = 0;
 };
 GraphStats graph_stats(const Csr& g);
+// GAP serialized graph (.sg) of g (export_sg.cpp); sym: the symmetrization, undirected
+void save_sg(const Csr& g, const std::string& path, bool sym);
 std::vector<uint64_t> log2_histogram(const std::vector<uint32_t>& deg);
 std::vector<uint32_t> out_degrees(const Csr& g);
 // Component id per vertex (BFS labeling in increasing root order) and the

@@ -188,6 +188,11 @@ int ggpu_degree_histogram(const ggpu_graph* g, int kind, uint64_t* hist, uint32_
     });
 }
 
+int ggpu_graph_save_sg(const ggpu_graph* g, const char* path, int sym) {
+    if (!g || !path) return fail(GGPU_EINVAL, "null argument");
+    return guarded([&] { ggpu::save_sg(g->g, path, sym != 0); });
+}
+
 int ggpu_graph_locality(const ggpu_graph* g, double* mean_gap, double* same_block64) {
     if (!g || !mean_gap || !same_block64) return fail(GGPU_EINVAL, "null argument");
     return guarded([&] {

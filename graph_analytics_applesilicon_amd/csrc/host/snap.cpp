@@ -11,6 +11,7 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -182,6 +183,40 @@ GraphStats graph_stats(const Csr& g) {
         }
     }
     return t;
+}
+
+// GAP Benchmark Suite serialized graph (src/graph/export_sg.cpp:1-12, the
+// layout of gapbs' WriteSerializedGraph): bool directed, int64 adjacency
+// entries, int64 vertices, int64 offsets[n+1], int32 neighbours[m], and for a
+// directed graph the in-CSR (its transpose) the same way. sym: write the
+// symmetrized graph as undirected (what BFS traverses).
+void save_sg(const Csr& g_in, const std::string& path, bool sym) {
+    if (g_in.n >= (1ull << 31)) throw std::invalid_argument("vertex ids do not fit GAP's int32 ids");
+    const Csr* g = &g_in;
+    Csr s;
+    if (sym && g_in.directed()) {
+        s = symmetrize(g_in);
+        g = &s;
+    }
+    const bool directed = !sym && g->directed();
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) throw std::runtime_error("cannot write " + path);
+    auto put_csr = [&](const Csr& c) {
+        std::vector<int64_t> wide;
+        for (uint64_t i = 0; i <= c.n; i += 1u << 20) {
+            const uint64_t e = std::min<uint64_t>(c.n + 1, i + (1u << 20));
+            wide.assign(c.rp.begin() + i, c.rp.begin() + e);
+            if (std::fwrite(wide.data(), 8, wide.size(), f) != wide.size()) return false;
+        }
+        return c.m == 0 || std::fwrite(c.ci.data(), 4, c.m, f) == c.m;
+    };
+    const int64_t m = (int64_t)g->m, n = (int64_t)g->n;
+    const uint8_t d = directed ? 1 : 0;
+    bool ok = std::fwrite(&d, 1, 1, f) == 1 && std::fwrite(&m, 8, 1, f) == 1 && std::fwrite(&n, 8, 1, f) == 1 &&
+              put_csr(*g);
+    if (ok && directed) ok = put_csr(transpose(*g));
+    ok = std::fclose(f) == 0 && ok;
+    if (!ok) throw std::runtime_error("short write to " + path);
 }
 
 // Bucket 0: degree 0; bucket k >= 1: degrees in [2^(k-1), 2^k). Trailing empty

@@ -89,6 +89,10 @@ int ggpu_graph_stats_compute(const ggpu_graph* g, ggpu_graph_stats* out);
  * kind 0 stored (out-) degrees, 1 in-degrees, 2 symmetrized degrees.
  * hist has room for 34 buckets; *buckets = used (trailing zeros dropped). */
 int ggpu_degree_histogram(const ggpu_graph* g, int kind, uint64_t* hist, uint32_t* buckets);
+/* gexport (src/graph/export_sg.cpp): the GAP Benchmark Suite .sg layout;
+ * sym != 0 writes the symmetrized graph as undirected, else a directed graph
+ * carries its transpose as the in-CSR. n must be below 2^31. */
+int ggpu_graph_save_sg(const ggpu_graph* g, const char* path, int sym);
 /* Largest connected component of a symmetric graph, relabeled in increasing
  * original-id order (row-major for grids); old_ids_out (optional, n entries)
  * receives the original id of each new vertex. */
