@@ -323,6 +323,11 @@ def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
     best_name = min(sweeps, key=sweeps.get)
     layout, order = combos[best_name]
     res["sweep_ms"] = sweeps
+    # gpu_push (FP64 atomics along the out-arcs) on the degree-ordered copy, for comparison
+    po = G.pr_options(iters=20, order=G.PR_ORDER_DEGREE)
+    pr.run("gpu_push", po)
+    res["push_sweep_ms"] = pr.run("gpu_push", po).gpu_ms / 20
+    log(f"  sweep (gpu_push, degree): {res['push_sweep_ms'] * 1e3:.1f} us")
     # Jacobi, certified stop: ||r_k - r*||_1 <= d/(1-d) * delta_k  =>  delta_k < 1e-4 * (1-d)/d
     tol_cert = 1e-4 * 0.15 / 0.85
     for variant, opts, key in (("gpu_pull_adaptive", G.pr_options(tol=tol_cert, iters=1000, order=order,
