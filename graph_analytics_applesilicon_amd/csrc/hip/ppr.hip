@@ -239,11 +239,15 @@ __global__ void k_ppr_unpermute(const float* src, float* dst, const uint32_t* id
     }
 }
 
-// One workgroup: column sums of the sweep's workgroup partials, in workgroup order.
+// Column sums of the sweep's workgroup partials: a wave per column (B + 1
+// columns: dangling mass per seed, then the L1 change), lane j adding
+// workgroups j, j + 64, ... and a fixed butterfly combining the lanes.
 __global__ void k_ppr_reduce(const double* partial, uint32_t blocks, uint32_t B, double* dang_out, double* delta_out) {
-    for (uint32_t col = threadIdx.x; col <= B; col += blockDim.x) {
-        double t = 0;
-        for (uint32_t blk = 0; blk < blocks; ++blk) t += partial[(size_t)blk * (B + 1) + col];
+    const uint32_t col = blockIdx.x, lane = threadIdx.x;
+    double t = 0;
+    for (uint32_t blk = lane; blk < blocks; blk += kWave) t += partial[(size_t)blk * (B + 1) + col];
+    t = wave_sum(t);
+    if (lane == 0) {
         if (col < B) dang_out[col] = t;
         else *delta_out = t;
     }
@@ -410,7 +414,7 @@ ggpu_pr_result ppr_run(PrEngine& e, const uint32_t* seeds_in, uint32_t B, const 
             case 3: ppr_launch<3>(A, grid, cgrid, o.tg, s); break;
             default: ppr_launch<4>(A, grid, cgrid, o.tg, s); break;
         }
-        k_ppr_reduce<<<1, 256, 0, s>>>(S.part.p, grid, B, S.dang.p + (size_t)nx * kPprMaxB, S.delta.p);
+        k_ppr_reduce<<<B + 1, kWave, 0, s>>>(S.part.p, grid, B, S.dang.p + (size_t)nx * kPprMaxB, S.delta.p);
         GGPU_HIP_TRY(hipGetLastError());
         if (!tol_mode) continue;
         GGPU_HIP_TRY(hipMemcpyAsync(pin, S.delta.p, 8, hipMemcpyDeviceToHost, s));
