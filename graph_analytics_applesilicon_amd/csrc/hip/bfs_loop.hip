@@ -1,6 +1,6 @@
 // Device-driven single-source BFS level loops (no reference counterpart; the
 // reference syncs the host every level, src/metal/bfs_gpu.mm:83-104):
-//   GGPU_LOOP_PERSISTENT  one cooperative launch runs every level; levels are
+//   GGPU_LOOP_PERSISTENT  one persistent launch runs every level; levels are
 //                         separated by the grid barrier of grid_sync.hpp;
 //   GGPU_LOOP_GRAPH       a captured hipGraph of K levels (a decide kernel and a
 //                         level kernel per level) is replayed until the
@@ -746,9 +746,14 @@ void BfsEngine::device_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, s
     GGPU_HIP_TRY(hipGetLastError());
     DlState out{};
     if (o.loop == GGPU_LOOP_PERSISTENT) {
+        // A plain launch: the barrier is the kernel's own (grid_sync.hpp), so it
+        // needs no grid-sync resource from the runtime (hipLaunchCooperativeKernel
+        // has failed here with "cannot be performed in the present state" while
+        // another process held the device). Residency: one workgroup per CU (its
+        // LDS footprint allows no second), grid = CUs; a workgroup that cannot
+        // start ends the launch through the bounded barrier (GGPU_ETIMEOUT).
         void* args[] = {&D};
-        GGPU_HIP_TRY(hipLaunchCooperativeKernel((const void*)k_dl_persistent, dim3(R.persist_grid), dim3(kDlThreads),
-                                                args, 0, s));
+        GGPU_HIP_TRY(hipLaunchKernel((const void*)k_dl_persistent, dim3(R.persist_grid), dim3(kDlThreads), args, 0, s));
         GGPU_HIP_TRY(hipEventRecord(R.ev[1], s));
         uint32_t abort = 0;
         GGPU_HIP_TRY(hipMemcpyAsync(&abort, D.abort, 4, hipMemcpyDeviceToHost, s));

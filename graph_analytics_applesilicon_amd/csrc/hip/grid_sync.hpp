@@ -1,4 +1,4 @@
-// Device-wide barrier for persistent cooperative kernels on gfx950.
+// Device-wide barrier for persistent kernels on gfx950 (every workgroup resident).
 //
 // Each XCD has its own L2, and a plain store stays in the writer's L2 until
 // it is written back, so a reader on another XCD can miss it. A fenced

@@ -1,5 +1,5 @@
 // Batched multi-source BFS (no reference counterpart): k <= 256 sources
-// traverse together in ONE persistent cooperative launch, one grid barrier
+// traverse together in ONE persistent launch (a workgroup per CU), one grid barrier
 // per level.
 //
 // State, W = ceil(k/64) u64 words per vertex, bit s%64 of word s/64 = source s:
@@ -964,7 +964,9 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         S.have = false;
         S.clean = false;  // until the launch is known to have finished
         GGPU_HIP_TRY(hipEventRecord(S.ev[0], s));
-        GGPU_HIP_TRY(hipLaunchCooperativeKernel(ms_kernel(W), dim3(G), dim3(kMsThreads), args, 0, s));
+        // plain launch, one workgroup per CU (the LDS footprint allows no second):
+        // the barrier is the kernel's own, bounded (see the single-source loop)
+        GGPU_HIP_TRY(hipLaunchKernel(ms_kernel(W), dim3(G), dim3(kMsThreads), args, 0, s));
         GGPU_HIP_TRY(hipEventRecord(S.ev[1], s));
         uint32_t ctl[16];
         GGPU_HIP_TRY(hipMemcpyAsync(ctl, S.ctl.p + kBarWords, sizeof ctl, hipMemcpyDeviceToHost, s));

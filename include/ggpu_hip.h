@@ -51,7 +51,7 @@ int ggpu_ctx_sync(ggpu_ctx* ctx);
 /* How the level loop is driven (no reference counterpart except HOST). */
 #define GGPU_LOOP_HOST 0       /* one host synchronization per level (the reference protocol) */
 #define GGPU_LOOP_GRAPH 1      /* device-side decisions; K levels per captured hipGraph launch */
-#define GGPU_LOOP_PERSISTENT 2 /* one cooperative kernel, device-wide barrier between phases */
+#define GGPU_LOOP_PERSISTENT 2 /* one persistent kernel (a workgroup per CU), device-wide barrier between phases */
 
 /* Who picks each level's action (top-down edge / top-down binned / bottom-up). */
 #define GGPU_POLICY_HEURISTIC 0 /* the variant's rule: GAP alpha/beta for gpu_do */
