@@ -209,6 +209,8 @@ def _optional_hip_symbols():
         "ggpu_mlp_loss_grad": ([vp, fp, fp, fp, C.POINTER(C.c_uint8), C.c_uint64, fp, fp], C.c_int),
         "ggpu_mlp_train": ([vp, fp, fp, fp, C.POINTER(C.c_uint8), C.c_uint64, C.c_uint32, C.c_float, fp],
                            C.c_int),
+        "ggpu_mlp_train_sgd": ([vp, fp, fp, fp, C.POINTER(C.c_uint8), C.c_uint64, C.c_uint32, C.c_float, C.c_float,
+                                fp], C.c_int),
         "ggpu_bfs_record_levels": ([vp, C.c_uint32, C.POINTER(BfsOpts), fp, fp, C.POINTER(Level), C.c_size_t,
                                     C.POINTER(C.c_size_t)], C.c_int),
         "ggpu_msbfs_record_levels": ([vp, _u32p, C.c_uint32, C.POINTER(BfsOpts), fp, fp, C.POINTER(Level),
@@ -751,6 +753,19 @@ def mlp_train(params, x, t, mask=None, epochs: int = 1000, lr: float = 1e-2, ctx
     c.check(c.lib.ggpu_mlp_train(c._h, _ptr(p, C.c_float), _ptr(xx, C.c_float), _ptr(tt, C.c_float),
                                  None if mk is None else _ptr(mk, C.c_uint8), len(xx), epochs, lr,
                                  C.byref(loss)))
+    return p, float(loss.value)
+
+
+def mlp_train_sgd(params, x, t, mask=None, epochs: int = 1000, lr: float = 1e-2, momentum: float = 0.0,
+                  ctx: Optional[Context] = None):
+    """Full-batch SGD with momentum on the GPU; returns (params, final loss)."""
+    c = ctx or default_context()
+    p = np.array(params, dtype=np.float32)
+    xx, tt, mk = _mlp_data(x, t, mask)
+    loss = C.c_float()
+    c.check(c.lib.ggpu_mlp_train_sgd(c._h, _ptr(p, C.c_float), _ptr(xx, C.c_float), _ptr(tt, C.c_float),
+                                     None if mk is None else _ptr(mk, C.c_uint8), len(xx), epochs, lr, momentum,
+                                     C.byref(loss)))
     return p, float(loss.value)
 
 

@@ -169,6 +169,16 @@ __global__ void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g, f
     p[q] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
 }
 
+// Stochastic gradient descent with momentum (full batch): v = mu v + g, p -= lr v.
+__global__ void k_mlp_sgd(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ v, float lr,
+                          float mu) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (uint32_t)kParams) return;
+    const float a = mu * v[q] + g[q];
+    v[q] = a;
+    p[q] -= lr * a;
+}
+
 // Device copies of one training set and the scratch of a backward pass.
 struct MlpData {
     DBuf<float> p, x, t, act, grad, m1, m2;
@@ -334,6 +344,35 @@ int ggpu_mlp_train(ggpu_ctx* ctx, float* params_inout, const float* x, const flo
             c2 *= 0.999;
             k_mlp_adam<<<pb, kMlpThreads, 0, s>>>(d.p.p, d.grad.p, d.m1.p, d.m2.p, lr, (float)(1 - c1),
                                                    (float)(1 - c2));
+            GGPU_HIP_TRY(hipGetLastError());
+        }
+        d.backward(s);  // loss at the final parameters
+        double l = 0;
+        GGPU_HIP_TRY(hipMemcpyAsync(params_inout, d.p.p, kParams * 4, hipMemcpyDeviceToHost, s));
+        GGPU_HIP_TRY(hipMemcpyAsync(&l, d.loss.p, 8, hipMemcpyDeviceToHost, s));
+        GGPU_HIP_TRY(hipStreamSynchronize(s));
+        if (final_loss) *final_loss = (float)l;
+    });
+}
+
+int ggpu_mlp_train_sgd(ggpu_ctx* ctx, float* params_inout, const float* x, const float* t, const uint8_t* mask,
+                       uint64_t count, uint32_t epochs, float lr, float momentum, float* final_loss) {
+    if (!ctx) return GGPU_EINVAL;
+    return guarded(ctx, [&] {
+        if (!params_inout || !x || !t || !count) throw ArgError("null argument or empty training set");
+        if (!(lr > 0) || !std::isfinite(lr)) throw ArgError("learning rate must be positive");
+        if (!(momentum >= 0 && momentum < 1)) throw ArgError("momentum must be in [0, 1)");
+        check_params(params_inout, GGPU_MLP_PARAMS);
+        GGPU_HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        MlpData d;
+        d.load(s, params_inout, x, t, mask, count);
+        d.m1.alloc(kParams);  // the velocity
+        GGPU_HIP_TRY(hipMemsetAsync(d.m1.p, 0, kParams * 4, s));
+        const uint32_t pb = (kParams + kMlpThreads - 1) / kMlpThreads;
+        for (uint32_t e = 0; e < epochs; ++e) {
+            d.backward(s);
+            k_mlp_sgd<<<pb, kMlpThreads, 0, s>>>(d.p.p, d.grad.p, d.m1.p, lr, momentum);
             GGPU_HIP_TRY(hipGetLastError());
         }
         d.backward(s);  // loss at the final parameters

@@ -228,6 +228,10 @@ int ggpu_mlp_loss_grad(ggpu_ctx* ctx, const float* params, const float* x, const
 /* GPU training: `epochs` full-batch Adam steps from params_inout. */
 int ggpu_mlp_train(ggpu_ctx* ctx, float* params_inout, const float* x, const float* t, const uint8_t* mask,
                    uint64_t count, uint32_t epochs, float lr, float* final_loss);
+/* GPU training: `epochs` full-batch SGD steps with momentum in [0, 1)
+ * (v = momentum * v + grad, params -= lr * v; momentum 0 is plain SGD). */
+int ggpu_mlp_train_sgd(ggpu_ctx* ctx, float* params_inout, const float* x, const float* t, const uint8_t* mask,
+                       uint64_t count, uint32_t epochs, float lr, float momentum, float* final_loss);
 /* Training data. The frontier and the counts of a level do not depend on
  * the actions taken before it, so the traversal is run once per action with
  * that action at every level: level l's three times are the three actions'

@@ -68,6 +68,28 @@ def test_adam_first_step_closed_form():
     assert np.all(np.abs(p1 - p) <= lr * 1.0001)
 
 
+def test_sgd_steps_match_oracle():
+    """One plain SGD step is p - lr g(p); two momentum steps are
+    p1 = p0 - lr g(p0), p2 = p1 - lr (mu g(p0) + g(p1)), with the FP64 oracle's
+    gradients."""
+    p = _params(11)
+    x, t = _data(400, 12)
+    x64, t64 = x.astype(np.float64), t.astype(np.float64)
+    lr, mu = 5e-3, 0.9
+    _, g0 = O.mlp_loss_grad(p.astype(np.float64), x64, t64)
+    p1, _ = G.mlp_train_sgd(p, x, t, epochs=1, lr=lr)
+    want1 = p.astype(np.float64) - lr * g0
+    np.testing.assert_allclose(p1, want1, rtol=0, atol=1e-6 + 1e-4 * lr * np.abs(g0).max())
+    _, g1 = O.mlp_loss_grad(want1, x64, t64)
+    p2, loss2 = G.mlp_train_sgd(p, x, t, epochs=2, lr=lr, momentum=mu)
+    want2 = want1 - lr * (mu * g0 + g1)
+    np.testing.assert_allclose(p2, want2, rtol=0, atol=2e-6 + 2e-4 * lr * np.abs(g0).max())
+    rl, _ = O.mlp_loss_grad(want2, x64, t64)
+    assert abs(loss2 - rl) <= 1e-4 * max(1.0, abs(rl))
+    with pytest.raises(ValueError):
+        G.mlp_train_sgd(p, x, t, epochs=1, lr=lr, momentum=1.0)
+
+
 def test_training_fits_a_learnable_target():
     rng = np.random.default_rng(3)
     x = rng.uniform(0, 1, (2000, 5)).astype(np.float32)
