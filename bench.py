@@ -366,6 +366,28 @@ def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
             l1, rel, s = O.check_ranks(ref, pr.ranks())
             res[key] = {"k": kstar, "ms": ms, "gpu_ms": r.gpu_ms, "l1_vs_fp64": l1, "max_rel_top1000": rel, "sum": s}
             log(f"  {key}: {kstar} sweeps, {ms:.2f} ms (L1 {l1:.2e})")
+    # the reference's cpu_omp PageRank (pagerank_cpu.cpp:40, OpenMP on every host core) beside
+    # the GPU: ms per iteration, and the same k* sweeps to L1 < 1e-4 (more if FP64 needs them)
+    if rlib is not None:
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            cores = os.cpu_count() or 1
+        t1 = time.perf_counter()
+        O.ref_pagerank(t.row_ptr, t.col_idx, od, 0.85, 5, 0.0, parallel=True)
+        per_it = (time.perf_counter() - t1) / 5 * 1e3
+        k = res.get("jacobi_kstar", {}).get("k", 1)
+        while k <= 60:
+            t1 = time.perf_counter()
+            rc, _, _ = O.ref_pagerank(t.row_ptr, t.col_idx, od, 0.85, k, 0.0, parallel=True)
+            ms = (time.perf_counter() - t1) * 1e3
+            l1, rel, s = O.check_ranks(ref, rc.astype(np.float32))
+            if l1 < 1e-4 and rel < 1e-3 and abs(s - 1) < 1e-4:
+                break
+            k += 1
+        res["cpu_omp_baseline"] = {"kind": "reference", "cores": cores, "ms_per_iteration": per_it,
+                                   "iters_to_l1_1e-4": k, "ms_to_l1_1e-4": ms, "l1_vs_fp64": l1}
+        log(f"  cpu_omp ({cores} cores): {per_it:.1f} ms/iteration, L1 < 1e-4 after {k} iterations in {ms:.1f} ms")
     per_sweep_ms = sweeps[best_name]
     ach = bytes_sweep / (per_sweep_ms / 1e3) / 1e9
     kname = "k_pr_sweep_hot" if layout == G.PR_LAYOUT_HOT else "k_pr_sweep"
