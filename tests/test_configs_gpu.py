@@ -1,5 +1,9 @@
 """Parity at the full sizes of the workload graphs (gen_named recipes):
 
+* rmat16_c1 (scale 16, ef 16, permuted, seed 1): gpu_do from all 64 pickSources
+  sources bit for bit against cpu_serial, the per-level (direction, frontier)
+  of cpu_do for eight of them; PageRank for 20 fixed FP32 iterations (scalar,
+  adaptive) against the oracle's 20 FP64 iterations.
 * grid_c3 (1024 x 2048 grid, p = 0.7, largest component, 2390 levels):
   gpu_do depths and per-level (direction, frontier) through the host,
   persistent and graph loops and hybrid; gpu_pull_scalar for 20 iterations.
@@ -30,6 +34,28 @@ def _pr64(rp, ci, od, iters, tol):
     if O.ref_lib() is not None:
         return O.ref_pagerank(rp, ci, od, 0.85, iters, tol, parallel=True)[0]
     return O.pagerank(rp, ci, od, 0.85, iters, tol)[0]
+
+
+def test_rmat16_c1_gpu_do_64_sources_and_pagerank_20():
+    g = G.Graph.named("rmat16_c1")
+    rp, ci = g.row_ptr, g.col_idx
+    bfs = G.GpuBfs(g)
+    src = g.pick_sources(64, 1)
+    assert len(src) == 64
+    for i, s in enumerate(src):
+        s = int(s)
+        r = bfs.run("gpu_do", s)
+        assert np.array_equal(r.dist, O.bfs_serial(rp, ci, s)), s
+        if i % 8 == 0:
+            _, levels = O.bfs_do_levels(rp, ci, s)
+            assert [(L.direction.decode(), L.frontier) for L in r.trace] == [(d, f) for d, f, _ in levels], s
+    od = g.degrees()
+    ref = O.pagerank(rp, ci, od, 0.85, 20, 0.0)[0]
+    pr = G.GpuPageRank(g, od)
+    for variant in ("gpu_pull_scalar", "gpu_pull_adaptive"):
+        res = pr.run(variant, G.pr_options(iters=20))
+        l1, rel, s = O.check_ranks(ref, pr.ranks())
+        assert res.iters == 20 and O.ranks_ok(l1, rel, s), (variant, l1, rel, s)
 
 
 def test_grid_c3_loops_and_scalar_pagerank():
