@@ -326,18 +326,30 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, MsAcc
 }
 
 // ----------------------------------------------------------------- top-down
-// Two phases. Push: every frontier edge v -> u ORs F_l[v] & ~visit[u] into
-// next[u] and sets u's bit in fb_{l+1} (fire-and-forget atomics; visit is
-// not written, so the filter is exact for earlier levels and duplicates of
-// this level merge in next). Collect (after a barrier): each marked vertex
+// Two phases. Push: every frontier edge v -> u ORs F_l[v] (less the bits
+// already in visit[u], when filtering) into next[u] and, when marking, sets
+// u's bit in fb_{l+1} (fire-and-forget atomics; visit is not written, so the
+// filter is exact for earlier levels and duplicates of this level merge in next). Collect (after a barrier): each marked vertex
 // takes its new bits, clears next, and fb_{l+1} is rewritten exactly.
 // kMark: also mark u in the next frontier bitmap fbn (the collect then visits
-// marked words only); without it the collect scans every vertex. Bits already
-// in next (a plain read; a stale one only costs the atomic) are not pushed
-// again: on a skewed graph most pushes of a big level land on a few hubs.
-template <int W, bool kMark>
+// marked words only); without it the collect scans every vertex. kFilter: bits
+// already in visit or next (plain reads; a stale one only costs the atomic) are
+// not pushed again, which pays once most pair-edges lead to reached vertices (on
+// a skewed graph most pushes of a late big level land on a few hubs); while
+// most are unreached the two reads are two random gathers per edge that filter
+// nothing, and every edge is pushed (the collect keeps next & ~visit either way).
+template <int W, bool kMark, bool kFilter>
 __device__ __forceinline__ void ms_push4(const MsDev& D, uint64_t* fbn, const uint32_t (&u)[4], uint32_t c,
                                          const uint64_t (&f)[W]) {
+    static_assert(kFilter || !kMark, "an unfiltered push marks nothing");
+    if (!kFilter) {
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i)
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (i < c && f[w]) atomicOr((unsigned long long*)&D.next[(size_t)u[i] * W + w], (unsigned long long)f[w]);
+        return;
+    }
     uint64_t vis[4][W], nxt[4][W];
 #pragma unroll
     for (uint32_t i = 0; i < 4; ++i)
@@ -363,7 +375,7 @@ __device__ __forceinline__ void ms_push4(const MsDev& D, uint64_t* fbn, const ui
 }
 
 // Pushes edges [b, e) of a vertex with frontier bits f, `step` lanes apart.
-template <int W, bool kMark>
+template <int W, bool kMark, bool kFilter>
 __device__ __forceinline__ void ms_push_range(const MsDev& D, uint64_t* fbn, uint32_t b, uint32_t e, uint32_t first,
                                               uint32_t step, const uint64_t (&f)[W]) {
     for (uint32_t x = b + first; x < e; x += 4 * step) {
@@ -375,11 +387,11 @@ __device__ __forceinline__ void ms_push_range(const MsDev& D, uint64_t* fbn, uin
             u[i] = y < e ? D.ci[y] : 0u;
             c += y < e;
         }
-        ms_push4<W, kMark>(D, fbn, u, c, f);
+        ms_push4<W, kMark, kFilter>(D, fbn, u, c, f);
     }
 }
 
-template <int W, bool kMark>
+template <int W, bool kMark, bool kFilter>
 __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex) {
     const uint32_t cs = level % D.C, ns = (level + 1) % D.C;
     const uint64_t* F = D.F + (size_t)cs * D.n * W;
@@ -406,7 +418,7 @@ __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex)
                         uint64_t f[W];
 #pragma unroll
                         for (int w = 0; w < W; ++w) f[w] = F[(size_t)v * W + w];
-                        ms_push_range<W, kMark>(D, fbn, b, e, 0, 1, f);
+                        ms_push_range<W, kMark, kFilter>(D, fbn, b, e, 0, 1, f);
                     }
                 }
             } else {
@@ -424,7 +436,7 @@ __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex)
                     uint64_t f[W];
 #pragma unroll
                     for (int w = 0; w < W; ++w) f[w] = F[(size_t)vj * W + w];
-                    ms_push_range<W, kMark>(D, fbn, bj, ej, lane, kWave, f);
+                    ms_push_range<W, kMark, kFilter>(D, fbn, bj, ej, lane, kWave, f);
                 }
             }
         }
@@ -447,7 +459,7 @@ __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex)
             uint64_t f[W];
 #pragma unroll
             for (int w = 0; w < W; ++w) f[w] = F[(size_t)v * W + w];
-            ms_push_range<W, kMark>(D, fbn, y, z, lane, kWave, f);
+            ms_push_range<W, kMark, kFilter>(D, fbn, y, z, lane, kWave, f);
         }
     }
 }
@@ -551,6 +563,32 @@ __device__ __forceinline__ void ms_clear(uint64_t* p, bool final) {
     else store_through(p, 0ull);
 }
 
+// Depth rows of sources s0 + b (b < ns) at one vertex: out[b * stride] =
+// base + (bit b of the six planes) where `any` has bit b, INF where
+// `unreached` has it too (a pair never reached has no plane bits). The loop
+// is unrolled over the 64 bits, so every shift count is a constant and a
+// source costs a few 32-bit bit-field operations; the store is streamed out,
+// as the traversal does not read the depth rows again.
+__device__ __forceinline__ void ms_write_depths(uint32_t* out, size_t stride, uint32_t ns, uint32_t base,
+                                                const uint64_t (&plane)[6], uint64_t unreached, uint64_t any) {
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t sh = 32 * h;
+        const uint32_t p0 = (uint32_t)(plane[0] >> sh), p1 = (uint32_t)(plane[1] >> sh);
+        const uint32_t p2 = (uint32_t)(plane[2] >> sh), p3 = (uint32_t)(plane[3] >> sh);
+        const uint32_t p4 = (uint32_t)(plane[4] >> sh), p5 = (uint32_t)(plane[5] >> sh);
+        const uint32_t un = (uint32_t)(unreached >> sh), an = (uint32_t)(any >> sh);
+#pragma unroll
+        for (uint32_t j = 0; j < 32; ++j) {
+            if (sh + j >= ns) return;
+            const uint32_t d = ((p0 >> j) & 1u) | (((p1 >> j) & 1u) << 1) | (((p2 >> j) & 1u) << 2) |
+                               (((p3 >> j) & 1u) << 3) | (((p4 >> j) & 1u) << 4) | (((p5 >> j) & 1u) << 5);
+            const uint32_t val = (base + d) | (0u - ((un >> j) & 1u));
+            if ((an >> j) & 1u) __builtin_nontemporal_store(val, out + (size_t)(sh + j) * stride);
+        }
+    }
+}
+
 template <int W>
 __device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool final) {
     const uint32_t lane = lane_id();
@@ -579,23 +617,10 @@ __device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool
                 unreached = ~D.visit[(size_t)v * W + w];
                 D.visit[(size_t)v * W + w] = 0ull;
             }
-            if (!(reached | unreached)) continue;
+            const uint64_t any = reached | unreached;
+            if (!__any(any != 0)) continue;
             const uint32_t s0 = (uint32_t)w * 64;
-            const uint32_t ns = min(64u, D.k - s0);
-            for (uint32_t b = 0; b < ns; ++b) {
-                uint32_t d;
-                if ((reached >> b) & 1ull) {
-                    d = base;
-#pragma unroll
-                    for (int p = 0; p < 6; ++p) d += (uint32_t)((plane[p] >> b) & 1ull) << p;
-                } else if ((unreached >> b) & 1ull) {
-                    d = GGPU_INF;
-                } else {
-                    continue;
-                }
-                // streamed out: the depth rows are not read again by the traversal
-                __builtin_nontemporal_store(d, &D.dist[(size_t)(s0 + b) * D.n + v]);
-            }
+            ms_write_depths(D.dist + (size_t)s0 * D.n + v, D.n, min(64u, D.k - s0), base, plane, unreached, any);
         }
         for (uint32_t i = 0; i < cnt; ++i) {
             const size_t slot = (base + i) % D.C;
@@ -729,8 +754,10 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
             // a big push marks nothing and the collect scans every vertex (fewer
             // memory operations than one contended bitmap atomic per push)
             const bool scan_all = (uint64_t)t.edges * 8 > (uint64_t)D.n;
-            if (scan_all) ms_td_push<W, false>(D, level, act == GGPU_ACTION_TD_BINNED);
-            else ms_td_push<W, true>(D, level, act == GGPU_ACTION_TD_BINNED);
+            const bool binned = act == GGPU_ACTION_TD_BINNED;
+            if (!scan_all) ms_td_push<W, true, true>(D, level, binned);
+            else if (pairs_u * 2 > total_pairs) ms_td_push<W, false, false>(D, level, binned);
+            else ms_td_push<W, false, true>(D, level, binned);
             if (!grid_barrier(D.bar, D.abort)) return;
             if (scan_all) ms_td_collect_all<W>(D, level, a);
             else ms_td_collect<W>(D, level, a);
