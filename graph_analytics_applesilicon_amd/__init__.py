@@ -37,6 +37,7 @@ BFS_VARIANTS = {"gpu_edge": BFS_EDGE, "gpu_binned": BFS_BINNED, "gpu_do": BFS_DO
                 "gpu_vertex": BFS_VERTEX, "gpu_naive": BFS_NAIVE}
 LOOP_HOST, LOOP_GRAPH, LOOP_PERSISTENT = 0, 1, 2
 POLICY_HEURISTIC, POLICY_LEARNED, POLICY_SCRIPTED = 0, 1, 2
+BATCH_ORDER_GIVEN, BATCH_ORDER_DEGREE = 0, 1  # batched BFS traversal numbering (depths in caller ids)
 ACTION_TD_EDGE, ACTION_TD_BINNED, ACTION_BU = 0, 1, 2
 PR_SCALAR, PR_VECTOR, PR_ADAPTIVE, PR_GS, PR_PUSH = 1, 2, 3, 4, 5
 PR_LAYOUT_STREAM, PR_LAYOUT_BINS, PR_LAYOUT_HOT = 0, 1, 2
@@ -66,7 +67,7 @@ class BfsOpts(C.Structure):
                 ("ept", C.c_uint32), ("alpha", C.c_double), ("beta", C.c_double),
                 ("hybrid_t", C.c_uint64), ("cpu_threads", C.c_int), ("loop", C.c_uint32),
                 ("policy", C.c_uint32), ("actions", C.POINTER(C.c_uint8)), ("n_actions", C.c_uint32),
-                ("graph_levels", C.c_uint32), ("time_levels", C.c_uint32)]
+                ("graph_levels", C.c_uint32), ("time_levels", C.c_uint32), ("batch_order", C.c_uint32)]
 
 
 class MsbfsResult(C.Structure):

@@ -19,6 +19,16 @@
 // has bit p) and written source-major with coalesced stores, when the C
 // slots are used up and at the end. That pass also clears what it read, so
 // visit / next / F / fb are all zero between launches.
+// A relabeled run traverses the graph renumbered by descending degree (the
+// frontier words most arcs gather then share a few MB of lines, which stay in
+// each XCD's L2): the depth pass writes a 64-byte record per vertex (planes,
+// unreached, any) in that numbering, and a separate kernel, k_ms_unpermute,
+// reads the record of perm[u] for every caller id u and writes u's depths.
+// Un-permute code inside the persistent kernel would cost its level loop
+// registers (scratch spills, ~10% slower levels), so a relabeled launch ends
+// when its C level slots are used up, after saving the loop state; the host
+// runs the un-permute of that window and launches the kernel again, which
+// resumes at the saved level.
 //
 // Cross-block data written inside a level goes through device-scope atomics
 // or write-through stores only (grid_sync.hpp), and every decision input
@@ -37,7 +47,25 @@ namespace ggpu {
 constexpr uint32_t kMsThreads = 1024;
 constexpr uint32_t kMsHeavy = 32;       // degree from which a vertex is handled as wave items
 constexpr uint32_t kMsChunk = 1024;     // edges per item
-constexpr uint32_t kMsSlots = 16;       // level slots (<= 64: six depth planes)
+constexpr uint32_t kMsSlots = 16;       // level slots: a window retires <= 15 levels (four depth planes)
+constexpr uint32_t kMsPlanes = 4;
+constexpr uint32_t kMsRec = 8;          // u64 words per (vertex, source word) depth record
+// Loop state of a relabeled run between launches (u64 words; zero before the first):
+enum : uint32_t {
+    kStResume = 0,   // 1: resume at kStLevel, whose action kStAct was decided by the previous launch
+    kStLevel,
+    kStAct,
+    kStBottomUp,     // heuristic policy state
+    kStPrevSize,
+    kStCounts,       // 3 words: levels run per action
+    kStCumPairs = kStCounts + 3,
+    kStBase,         // first level of the window whose records the un-permute reads
+    kStDone,         // 1: the traversal has ended
+    kStLoopTicks,    // device clock spent in the level loop by earlier launches
+    kStSetupTicks,
+    kMsState
+};
+static_assert(kMsSlots <= 1u << kMsPlanes, "depth offsets within a window fit the planes");
 constexpr uint32_t kMsTraceCap = 1u << 16;
 constexpr uint32_t kPartWords = 4;
 // Bottom-up LDS cache: the frontier words of the highest-degree vertices
@@ -59,6 +87,9 @@ struct MsDev {
     uint32_t* dist;                // k * n, source-major
     const uint32_t* sources;       // k
     const uint4* heavy;            // {v, first edge, end edge, degree}
+    const uint32_t* perm;          // relabeled runs: perm[caller id] = traversal id (else null)
+    uint64_t* rec;                 // relabeled runs: n * W depth records (planes, unreached, any)
+    unsigned long long* state;     // kMsState words: the loop state between launches of one run
     uint32_t* bar;                 // kBarWords
     uint32_t* abort;
     uint32_t* out;                 // {levels, 0, actions[3], 0, clock at start, level 0, loop end, finish}
@@ -564,30 +595,31 @@ __device__ __forceinline__ void ms_clear(uint64_t* p, bool final) {
 }
 
 // Depth rows of sources s0 + b (b < ns) at one vertex: out[b * stride] =
-// base + (bit b of the six planes) where `any` has bit b, INF where
+// base + (bit b of the four planes) where `any` has bit b, INF where
 // `unreached` has it too (a pair never reached has no plane bits). The loop
 // is unrolled over the 64 bits, so every shift count is a constant and a
 // source costs a few 32-bit bit-field operations; the store is streamed out,
 // as the traversal does not read the depth rows again.
 __device__ __forceinline__ void ms_write_depths(uint32_t* out, size_t stride, uint32_t ns, uint32_t base,
-                                                const uint64_t (&plane)[6], uint64_t unreached, uint64_t any) {
+                                                const uint64_t (&plane)[kMsPlanes], uint64_t unreached, uint64_t any) {
 #pragma unroll
     for (uint32_t h = 0; h < 2; ++h) {
         const uint32_t sh = 32 * h;
         const uint32_t p0 = (uint32_t)(plane[0] >> sh), p1 = (uint32_t)(plane[1] >> sh);
         const uint32_t p2 = (uint32_t)(plane[2] >> sh), p3 = (uint32_t)(plane[3] >> sh);
-        const uint32_t p4 = (uint32_t)(plane[4] >> sh), p5 = (uint32_t)(plane[5] >> sh);
         const uint32_t un = (uint32_t)(unreached >> sh), an = (uint32_t)(any >> sh);
 #pragma unroll
         for (uint32_t j = 0; j < 32; ++j) {
             if (sh + j >= ns) return;
             const uint32_t d = ((p0 >> j) & 1u) | (((p1 >> j) & 1u) << 1) | (((p2 >> j) & 1u) << 2) |
-                               (((p3 >> j) & 1u) << 3) | (((p4 >> j) & 1u) << 4) | (((p5 >> j) & 1u) << 5);
+                               (((p3 >> j) & 1u) << 3);
             const uint32_t val = (base + d) | (0u - ((un >> j) & 1u));
             if ((an >> j) & 1u) __builtin_nontemporal_store(val, out + (size_t)(sh + j) * stride);
         }
     }
 }
+
+typedef unsigned long long ms_u64x2 __attribute__((ext_vector_type(2)));
 
 template <int W>
 __device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool final) {
@@ -602,14 +634,14 @@ __device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool
         }
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            uint64_t plane[6] = {0, 0, 0, 0, 0, 0}, reached = 0;
+            uint64_t plane[kMsPlanes] = {0, 0, 0, 0}, reached = 0;
             // uniform over the window's levels: each load is one coalesced
             // (masked) access across the wave's 64 consecutive vertices
             for (uint32_t i = 0; i < cnt; ++i) {
                 const uint64_t f = ((levels >> i) & 1ull) ? D.F[((size_t)((base + i) % D.C) * D.n + v) * W + w] : 0ull;
                 reached |= f;
 #pragma unroll
-                for (int p = 0; p < 6; ++p)
+                for (uint32_t p = 0; p < kMsPlanes; ++p)
                     if ((i >> p) & 1u) plane[p] |= f;
             }
             uint64_t unreached = 0;
@@ -618,6 +650,16 @@ __device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool
                 D.visit[(size_t)v * W + w] = 0ull;
             }
             const uint64_t any = reached | unreached;
+            if (D.perm) {
+                // the record of (v, w): plain 16-byte stores, released before the
+                // barrier that precedes their reads
+                ms_u64x2* r = (ms_u64x2*)(D.rec + ((size_t)v * W + w) * kMsRec);
+                r[0] = ms_u64x2{plane[0], plane[1]};
+                r[1] = ms_u64x2{plane[2], plane[3]};
+                r[2] = ms_u64x2{unreached, any};
+                r[3] = ms_u64x2{0ull, 0ull};  // whole 64-byte lines: no partial-line writes
+                continue;
+            }
             if (!__any(any != 0)) continue;
             const uint32_t s0 = (uint32_t)w * 64;
             ms_write_depths(D.dist + (size_t)s0 * D.n + v, D.n, min(64u, D.k - s0), base, plane, unreached, any);
@@ -631,6 +673,27 @@ __device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool
             }
             // the whole wave has read the frontier word of its 64-vertex group
             if (lane == 0) ms_clear<W>(&D.fb[slot * D.nb + wd], final);
+        }
+    }
+}
+
+// Relabeled runs, after the launch that retired a window: caller id u takes
+// the record of perm[u] (one random 48-byte read) and writes its depth rows
+// with the wave's 64 consecutive u, as the direct depth pass does.
+template <int W>
+__global__ void __launch_bounds__(256) k_ms_unpermute(MsDev D) {
+    const uint32_t base = (uint32_t)D.state[kStBase];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < D.n; u += stride) {
+        const uint32_t v = D.perm[u];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const ms_u64x2* r = (const ms_u64x2*)(D.rec + ((size_t)v * W + w) * kMsRec);
+            const ms_u64x2 a = r[0], b = r[1], c = r[2];
+            const uint64_t plane[kMsPlanes] = {a.x, a.y, b.x, b.y};
+            if (!__any(c.y != 0)) continue;
+            const uint32_t s0 = (uint32_t)w * 64;
+            ms_write_depths(D.dist + (size_t)s0 * D.n + u, D.n, min(64u, D.k - s0), base, plane, c.x, c.y);
         }
     }
 }
@@ -681,10 +744,13 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
         for (uint32_t i = threadIdx.x; i < GGPU_MLP_PARAMS; i += blockDim.x) s_mlp[i] = D.mlp[i];
     uint64_t clk[4] = {0, 0, 0, 0};
     if (blockIdx.x == 0 && threadIdx.x == 0) clk[0] = wall_clock64();
-    MsAcc acc;
-    ms_seed<W>(D, acc);
-    ms_flush(D, 0, acc);
-    if (!grid_barrier(D.bar, D.abort)) return;
+    const bool resume = D.state && D.state[kStResume];
+    if (!resume) {
+        MsAcc acc;
+        ms_seed<W>(D, acc);
+        ms_flush(D, 0, acc);
+        if (!grid_barrier(D.bar, D.abort)) return;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) clk[1] = wall_clock64();
 
     MsPolicyState ps;
@@ -694,57 +760,90 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
     const unsigned long long total_pairs = (unsigned long long)D.k * D.m;
     uint64_t t_prev = 0;
     unsigned long long cum_pairs = 0;  // identical in every thread: t is the same everywhere
+    uint32_t pending = 0xFFFFFFFFu;    // a resumed level's action, decided by the previous launch
+    if (resume) {
+        level = base = (uint32_t)D.state[kStLevel];
+        pending = (uint32_t)D.state[kStAct];
+        ps.bottom_up = (uint32_t)D.state[kStBottomUp];
+        ps.prev_size = D.state[kStPrevSize];
+        for (int i = 0; i < 3; ++i) counts[i] = (uint32_t)D.state[kStCounts + i];
+        cum_pairs = D.state[kStCumPairs];
+        t_prev = clk[1];
+    }
     for (;; ++level) {
         const MsAcc t = ms_totals(D, level & 1);
-        cum_pairs += t.pairs;
+        if (pending == 0xFFFFFFFFu) cum_pairs += t.pairs;
         const uint64_t pairs_u = total_pairs > cum_pairs ? total_pairs - cum_pairs : 0;
-        float y[3] = {0.f, 0.f, 0.f};
-        if (D.policy == GGPU_POLICY_LEARNED && D.mlp && threadIdx.x < kWave && t.size) {
-            // the level boundary's inference: wave 0 evaluates the MLP on this level's features
-            float f[5];
-            ms_features(D, level, t, pairs_u, f);
-            mlp_forward_wave(s_mlp, f, y);
-        }
-        if (threadIdx.x == 0) {
-            if (blockIdx.x == 0) {
-                const uint64_t now = wall_clock64();
-                if (level > 0 && level - 1 < D.trace_cap) {
-                    if (prev_action == GGPU_ACTION_BU) D.trace[level - 1].edges = t.examined;
-                    D.trace[level - 1].time_ms = (double)(now - t_prev) * D.tick_ms;
-                }
-                t_prev = now;
+        uint32_t act = pending;
+        if (pending == 0xFFFFFFFFu) {
+            float y[3] = {0.f, 0.f, 0.f};
+            if (D.policy == GGPU_POLICY_LEARNED && D.mlp && threadIdx.x < kWave && t.size) {
+                // the level boundary's inference: wave 0 evaluates the MLP on this level's features
+                float f[5];
+                ms_features(D, level, t, pairs_u, f);
+                mlp_forward_wave(s_mlp, f, y);
             }
-            uint32_t act = 0xFFFFFFFFu;
-            if (t.size) {
-                act = ms_choose(D, y, level, t, pairs_u, ps);
-                counts[act]++;
-                if (blockIdx.x == 0 && level < D.trace_cap) {
-                    ggpu_level L{};
-                    L.level = level;
-                    L.action = act;
-                    L.frontier = t.size;
-                    L.edges = t.edges;
-                    L.frontier_edges = t.edges;
-                    L.unvisited_edges = (uint64_t)((double)pairs_u / (double)D.k);
-                    L.direction = act == GGPU_ACTION_BU ? 'B' : 'T';
-                    L.device = 'G';
-                    D.trace[level] = L;
-                    if (D.feat) ms_features(D, level, t, pairs_u, D.feat + (size_t)level * 5);
+            if (threadIdx.x == 0) {
+                if (blockIdx.x == 0) {
+                    const uint64_t now = wall_clock64();
+                    if (level > 0 && level - 1 < D.trace_cap) {
+                        if (prev_action == GGPU_ACTION_BU) D.trace[level - 1].edges = t.examined;
+                        D.trace[level - 1].time_ms = (double)(now - t_prev) * D.tick_ms;
+                    }
+                    t_prev = now;
                 }
+                uint32_t a = 0xFFFFFFFFu;
+                if (t.size) {
+                    a = ms_choose(D, y, level, t, pairs_u, ps);
+                    counts[a]++;
+                    if (blockIdx.x == 0 && level < D.trace_cap) {
+                        ggpu_level L{};
+                        L.level = level;
+                        L.action = a;
+                        L.frontier = t.size;
+                        L.edges = t.edges;
+                        L.frontier_edges = t.edges;
+                        L.unvisited_edges = (uint64_t)((double)pairs_u / (double)D.k);
+                        L.direction = a == GGPU_ACTION_BU ? 'B' : 'T';
+                        L.device = 'G';
+                        D.trace[level] = L;
+                        if (D.feat) ms_features(D, level, t, pairs_u, D.feat + (size_t)level * 5);
+                    }
+                }
+                s_act = a;
             }
-            s_act = act;
+            __syncthreads();
+            act = s_act;
+            __syncthreads();
+            if (act == 0xFFFFFFFFu) break;
+            if (level + 1 - base == D.C) {
+                // slot (level + 1) % C still holds level `base`: retire levels [base, level)
+                ms_materialize<W>(D, base, level - base, false);
+                if (D.perm) {
+                    // relabeled: the window's records are written; end the launch
+                    // and let the host run their un-permute, then resume here
+                    if (blockIdx.x == 0 && threadIdx.x == 0) {
+                        unsigned long long* st = D.state;
+                        st[kStResume] = 1;
+                        st[kStLevel] = level;
+                        st[kStAct] = act;
+                        st[kStBottomUp] = ps.bottom_up;
+                        st[kStPrevSize] = ps.prev_size;
+                        for (int i = 0; i < 3; ++i) st[kStCounts + i] = counts[i];
+                        st[kStCumPairs] = cum_pairs;
+                        st[kStBase] = base;
+                        st[kStDone] = 0;
+                        st[kStLoopTicks] += wall_clock64() - clk[1];
+                        if (!resume) st[kStSetupTicks] = clk[1] - clk[0];
+                    }
+                    return;
+                }
+                if (!grid_barrier(D.bar, D.abort)) return;
+                base = level;
+            }
         }
-        __syncthreads();
-        const uint32_t act = s_act;
-        __syncthreads();
-        if (act == 0xFFFFFFFFu) break;
+        pending = 0xFFFFFFFFu;
         prev_action = act;
-        if (level + 1 - base == D.C) {
-            // slot (level + 1) % C still holds level `base`: retire levels [base, level)
-            ms_materialize<W>(D, base, level - base, false);
-            if (!grid_barrier(D.bar, D.abort)) return;
-            base = level;
-        }
         MsAcc a;
         if (act == GGPU_ACTION_BU) {
             // the hot copy (~25 us per level on 256 workgroups) pays while at least
@@ -775,6 +874,15 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
         D.out[3] = counts[1];
         D.out[4] = counts[2];
         for (int i = 0; i < 3; ++i) D.out[6 + i] = (uint32_t)(clk[i + 1] - clk[i]);
+        if (D.state) {
+            // a resumed run: setup of the first launch, level loops of all of them
+            if (resume) {
+                D.out[6] = (uint32_t)D.state[kStSetupTicks];
+                D.out[7] += (uint32_t)D.state[kStLoopTicks];
+            }
+            D.state[kStBase] = base;
+            D.state[kStDone] = 1;
+        }
     }
 }
 
@@ -786,6 +894,15 @@ struct MsState {
     DBuf<uint4> heavy;
     DBuf<uint32_t> cib, hot;  // bottom-up col_idx with hot slots, hot ids (for hot_w words per vertex)
     uint32_t hot_w = 0, n_hot = 0;
+    // the numbering heavy / cib / hot were built in; for DEGREE the renumbered
+    // graph (host copy for rebuilding them, device copy to traverse), perm[caller
+    // id] = traversal id, and the depth records
+    uint32_t order = 0xFFFFFFFFu;
+    Csr relabeled;
+    std::vector<uint32_t> perm_host;
+    DBuf<uint32_t> rp2, ci2, perm;
+    DBuf<uint64_t> rec;
+    DBuf<unsigned long long> st;  // kMsState loop-state words
     DBuf<ggpu_level> trace;
     DBuf<float> feat;
     DBuf<uint8_t> actions;
@@ -821,6 +938,19 @@ static const void* ms_kernel(uint32_t W) {
     }
 }
 
+static const void* ms_unpermute_kernel(uint32_t W) {
+    switch (W) {
+        case 1: return (const void*)k_ms_unpermute<1>;
+        case 2: return (const void*)k_ms_unpermute<2>;
+        case 3: return (const void*)k_ms_unpermute<3>;
+        default: return (const void*)k_ms_unpermute<4>;
+    }
+}
+// enough 256-lane blocks to cover n, at most 16 per CU (grid-stride beyond)
+static uint32_t ms_unpermute_grid(ggpu_ctx* ctx, uint32_t n) {
+    const uint64_t want = ((uint64_t)n + 255) / 256;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->cus * 16));
+}
 static uint32_t ms_grid(ggpu_ctx* ctx, MsState& S, uint32_t W) {
     if (!S.grid[W]) {
         int per_cu = 0;
@@ -857,9 +987,34 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         if (o.policy == GGPU_POLICY_SCRIPTED && (!o.actions || !o.n_actions))
             throw ArgError("scripted policy needs actions");
         if (!(o.alpha > 0) || !(o.beta > 0)) throw ArgError("alpha and beta must be positive");
+        if (o.batch_order > GGPU_BATCH_ORDER_DEGREE) throw ArgError("unknown batch_order");
         GGPU_HIP_TRY(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
         MsState& S = ms_state(e);
+        const bool relabel = o.batch_order == GGPU_BATCH_ORDER_DEGREE && e.n > 0;
+        if (S.order != o.batch_order) {
+            S.heavy_ready = false;
+            S.hot_w = 0;
+            S.relabeled = Csr();
+            S.perm_host.clear();
+            S.rp2.release();
+            S.ci2.release();
+            S.perm.release();
+            S.rec.release();
+            if (relabel) {
+                S.perm_host = order_perm(e.host, "degree", 1);
+                S.relabeled = permute(e.host, S.perm_host);
+                S.rp2.alloc(S.relabeled.rp.size());
+                S.ci2.alloc(S.relabeled.ci.size() + 1);
+                S.perm.alloc(e.n);
+                GGPU_HIP_TRY(hipMemcpy(S.rp2.p, S.relabeled.rp.data(), S.relabeled.rp.size() * 4, hipMemcpyHostToDevice));
+                if (!S.relabeled.ci.empty())
+                    GGPU_HIP_TRY(hipMemcpy(S.ci2.p, S.relabeled.ci.data(), S.relabeled.ci.size() * 4, hipMemcpyHostToDevice));
+                GGPU_HIP_TRY(hipMemcpy(S.perm.p, S.perm_host.data(), (size_t)e.n * 4, hipMemcpyHostToDevice));
+            }
+            S.order = o.batch_order;
+        }
+        const Csr& hg = relabel ? S.relabeled : e.host;
         const uint32_t W = (k + 63) / 64;
         const uint32_t n = e.n;
         const uint32_t nb = (n + 63) / 64;
@@ -871,10 +1026,10 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         if (!S.heavy_ready) {
             std::vector<uint4> items;
             for (uint32_t v = 0; v < n; ++v) {
-                const uint32_t dg = e.host.deg(v);
+                const uint32_t dg = hg.deg(v);
                 if (dg < kMsHeavy) continue;
-                for (uint32_t x = e.host.rp[v]; x < e.host.rp[v + 1]; x += kMsChunk)
-                    items.push_back(make_uint4(v, x, std::min(x + kMsChunk, e.host.rp[v + 1]), dg));
+                for (uint32_t x = hg.rp[v]; x < hg.rp[v + 1]; x += kMsChunk)
+                    items.push_back(make_uint4(v, x, std::min(x + kMsChunk, hg.rp[v + 1]), dg));
             }
             S.n_heavy = (uint32_t)items.size();
             S.heavy.alloc(items.size() + 1);
@@ -888,14 +1043,14 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             std::vector<uint32_t> order(n);
             for (uint32_t v = 0; v < n; ++v) order[v] = v;
             std::partial_sort(order.begin(), order.begin() + H, order.end(), [&](uint32_t a, uint32_t b) {
-                const uint32_t da = e.host.deg(a), db = e.host.deg(b);
+                const uint32_t da = hg.deg(a), db = hg.deg(b);
                 return da != db ? da > db : a < b;
             });
             std::vector<uint32_t> slot(n, 0xFFFFFFFFu);
             for (uint32_t i = 0; i < H; ++i) slot[order[i]] = i;
-            std::vector<uint32_t> cib(e.host.ci.size());
+            std::vector<uint32_t> cib(hg.ci.size());
             for (size_t x = 0; x < cib.size(); ++x) {
-                const uint32_t u = e.host.ci[x];
+                const uint32_t u = hg.ci[x];
                 cib[x] = slot[u] != 0xFFFFFFFFu ? (kMsHotBit | slot[u]) : u;
             }
             S.cib.alloc(cib.size() + 1);
@@ -936,6 +1091,10 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         const uint32_t G = ms_grid(ctx, S, W);
         S.part.alloc((size_t)2 * G * kPartWords);
         S.dist.alloc((size_t)k * n);
+        if (relabel) {
+            S.rec.alloc(words * kMsRec);
+            S.st.alloc(kMsState);
+        }
         S.k = k;
         S.sources.alloc(k);
         S.ctl.alloc(kBarWords + 16);
@@ -947,12 +1106,18 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             GGPU_HIP_TRY(hipMemcpyAsync(S.actions.p, o.actions, o.n_actions, hipMemcpyHostToDevice, s));
         }
         const bool learned = o.policy == GGPU_POLICY_LEARNED && ctx->policy_set[GGPU_POLICY_BATCHED];
-        GGPU_HIP_TRY(hipMemcpyAsync(S.sources.p, sources, 4ull * k, hipMemcpyHostToDevice, s));
+        std::vector<uint32_t> src(sources, sources + k);
+        if (relabel)
+            for (uint32_t& x : src) x = S.perm_host[x];
+        GGPU_HIP_TRY(hipMemcpy(S.sources.p, src.data(), 4ull * k, hipMemcpyHostToDevice));
         GGPU_HIP_TRY(hipMemsetAsync(S.ctl.p, 0, (kBarWords + 16) * sizeof(uint32_t), s));
 
         MsDev D{};
-        D.rp = e.rp.p;
-        D.ci = e.ci.p;
+        D.rp = relabel ? S.rp2.p : e.rp.p;
+        D.ci = relabel ? S.ci2.p : e.ci.p;
+        D.perm = relabel ? S.perm.p : nullptr;
+        D.rec = relabel ? S.rec.p : nullptr;
+        D.state = relabel ? S.st.p : nullptr;
         D.cib = S.cib.p;
         D.hot = S.hot.p;
         D.n_hot = S.n_hot;
@@ -990,15 +1155,29 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         void* args[] = {&D};
         S.have = false;
         S.clean = false;  // until the launch is known to have finished
+        if (relabel) GGPU_HIP_TRY(hipMemsetAsync(S.st.p, 0, kMsState * 8, s));
         GGPU_HIP_TRY(hipEventRecord(S.ev[0], s));
-        // plain launch, one workgroup per CU (the LDS footprint allows no second):
-        // the barrier is the kernel's own, bounded (see the single-source loop)
-        GGPU_HIP_TRY(hipLaunchKernel(ms_kernel(W), dim3(G), dim3(kMsThreads), args, 0, s));
-        GGPU_HIP_TRY(hipEventRecord(S.ev[1], s));
+        // plain launches, one workgroup per CU (the LDS footprint allows no second):
+        // the barrier is the kernel's own, bounded (see the single-source loop).
+        // A relabeled run takes one launch per window of C levels, each followed
+        // by the un-permute of that window's depth records.
         uint32_t ctl[16];
-        GGPU_HIP_TRY(hipMemcpyAsync(ctl, S.ctl.p + kBarWords, sizeof ctl, hipMemcpyDeviceToHost, s));
-        GGPU_HIP_TRY(hipStreamSynchronize(s));
-        if (ctl[0]) throw ArgError("batched BFS: grid barrier timed out", GGPU_ETIMEOUT);
+        for (uint32_t launch = 0;; ++launch) {
+            if (launch) GGPU_HIP_TRY(hipMemsetAsync(S.ctl.p, 0, kBarWords * sizeof(uint32_t), s));
+            GGPU_HIP_TRY(hipLaunchKernel(ms_kernel(W), dim3(G), dim3(kMsThreads), args, 0, s));
+            if (relabel)
+                GGPU_HIP_TRY(hipLaunchKernel(ms_unpermute_kernel(W), dim3(ms_unpermute_grid(ctx, n)), dim3(256),
+                                             args, 0, s));
+            GGPU_HIP_TRY(hipEventRecord(S.ev[1], s));  // the last launch's record is the one timed
+            unsigned long long done = 1;
+            if (relabel)
+                GGPU_HIP_TRY(hipMemcpyAsync(&done, S.st.p + kStDone, 8, hipMemcpyDeviceToHost, s));
+            GGPU_HIP_TRY(hipMemcpyAsync(ctl, S.ctl.p + kBarWords, sizeof ctl, hipMemcpyDeviceToHost, s));
+            GGPU_HIP_TRY(hipStreamSynchronize(s));
+            if (ctl[0]) throw ArgError("batched BFS: grid barrier timed out", GGPU_ETIMEOUT);
+            if (done) break;
+        }
+
         S.clean = true;
         float ms = 0;
         GGPU_HIP_TRY(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));

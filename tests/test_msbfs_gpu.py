@@ -55,15 +55,19 @@ MODES = {
     "all-bottom-up": dict(alpha=1e12, beta=1e12),
     "all-top-down": dict(alpha=1e-9),
 }
+# traversal numbering: renumbered by degree (the default; depths are mapped
+# back to the caller's ids) and the caller's own
+ORDERS = {"degree": G.BATCH_ORDER_DEGREE, "given": G.BATCH_ORDER_GIVEN}
 
 
+@pytest.mark.parametrize("order", list(ORDERS))
 @pytest.mark.parametrize("name", SUITE)
 @pytest.mark.parametrize("k", [1, 5, 64, 100, 256])
-def test_batched_matches_serial_per_source(name, k):
+def test_batched_matches_serial_per_source(name, k, order):
     g, bfs = _graph(name)
     src = _batch(g, k)
     for mode, kw in MODES.items():
-        _check(g, bfs, src, G.bfs_options(**kw), f"{name} k={k} {mode}")
+        _check(g, bfs, src, G.bfs_options(batch_order=ORDERS[order], **kw), f"{name} k={k} {mode} {order}")
 
 
 @pytest.mark.parametrize("name", ["rmat12", "grid100x100", "path1000", "star100k"])
@@ -75,8 +79,10 @@ def test_batched_scripted_actions(name):
         src = _batch(g, k, seed=k)
         for trial in range(4):
             actions = rng.integers(0, 3, size=int(rng.integers(1, 9))).tolist()
-            res, trace = _check(g, bfs, src, G.bfs_options(policy=G.POLICY_SCRIPTED, actions=actions),
-                                f"{name} k={k} actions={actions}")
+            order = list(ORDERS)[trial % 2]
+            res, trace = _check(g, bfs, src, G.bfs_options(policy=G.POLICY_SCRIPTED, actions=actions,
+                                                           batch_order=ORDERS[order]),
+                                f"{name} k={k} actions={actions} {order}")
             for L in trace[: len(trace)]:
                 assert L.action == actions[L.level % len(actions)]
 
@@ -92,6 +98,10 @@ def test_batched_all_same_source_and_state_reuse():
     d1, _, _ = bfs.run_batch(src)
     d2, _, _ = bfs.run_batch(src)
     assert np.array_equal(d1, d2)
+    # switching the numbering back and forth on one object rebuilds its arrays
+    for order in ("given", "degree", "given", "degree"):
+        d3, _, _ = bfs.run_batch(src, G.bfs_options(batch_order=ORDERS[order]))
+        assert np.array_equal(d1, d3), order
 
 
 def test_batched_larger_graph():
@@ -104,7 +114,8 @@ def test_batched_larger_graph():
     for k in (64, 256):
         src = np.array(g.pick_sources(k, 1), dtype=np.uint32)
         for mode, kw in MODES.items():
-            _check(g, bfs, src, G.bfs_options(**kw), f"rmat16_c1 k={k} {mode}")
+            for order, o in ORDERS.items():
+                _check(g, bfs, src, G.bfs_options(batch_order=o, **kw), f"rmat16_c1 k={k} {mode} {order}")
 
 
 def test_batched_errors():
@@ -118,6 +129,8 @@ def test_batched_errors():
         fresh.run_batch(np.array([g.n], dtype=np.uint32))
     with pytest.raises(ValueError):
         fresh.run_batch(np.array([0], dtype=np.uint32), G.bfs_options(policy=G.POLICY_SCRIPTED))
+    with pytest.raises(ValueError):
+        fresh.run_batch(np.array([0], dtype=np.uint32), G.bfs_options(batch_order=2))
     # nothing has run on `fresh`: fetching is a state error
     out = np.zeros(g.n, dtype=np.uint32)
     rc = fresh.ctx.lib.ggpu_msbfs_fetch_dist(fresh._h, out.ctypes.data_as(G.C.POINTER(G.C.c_uint32)))
