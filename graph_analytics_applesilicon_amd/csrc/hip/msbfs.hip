@@ -21,8 +21,8 @@
 // visit / next / F / fb are all zero between launches.
 // A relabeled run traverses the graph renumbered by descending degree (the
 // frontier words most arcs gather then share a few MB of lines, which stay in
-// each XCD's L2): the depth pass writes a 64-byte record per vertex (planes,
-// unreached, any) in that numbering, and a separate kernel, k_ms_unpermute,
+// each XCD's L2): the depth pass writes a 32-byte record per vertex (four
+// planes, a 4-bit code per source) in that numbering, and a separate kernel, k_ms_unpermute,
 // reads the record of perm[u] for every caller id u and writes u's depths.
 // Un-permute code inside the persistent kernel would cost its level loop
 // registers (scratch spills, ~10% slower levels), so a relabeled launch ends
@@ -47,9 +47,12 @@ namespace ggpu {
 constexpr uint32_t kMsThreads = 1024;
 constexpr uint32_t kMsHeavy = 32;       // degree from which a vertex is handled as wave items
 constexpr uint32_t kMsChunk = 1024;     // edges per item
-constexpr uint32_t kMsSlots = 16;       // level slots: a window retires <= 15 levels (four depth planes)
+constexpr uint32_t kMsSlots = 15;       // level slots: a window retires <= 14 levels (four depth planes)
 constexpr uint32_t kMsPlanes = 4;
-constexpr uint32_t kMsRec = 8;          // u64 words per (vertex, source word) depth record
+constexpr uint32_t kMsRec = 4;          // u64 words per (vertex, source word) depth record: the planes
+// Codes of a depth record (the four plane bits of a source): 0..13 depth - base,
+// kCodeSkip: not reached in this window (an earlier window wrote it), kCodeInf: never reached.
+constexpr uint32_t kCodeSkip = 14, kCodeInf = 15;
 // Loop state of a relabeled run between launches (u64 words; zero before the first):
 enum : uint32_t {
     kStResume = 0,   // 1: resume at kStLevel, whose action kStAct was decided by the previous launch
@@ -65,7 +68,7 @@ enum : uint32_t {
     kStSetupTicks,
     kMsState
 };
-static_assert(kMsSlots <= 1u << kMsPlanes, "depth offsets within a window fit the planes");
+static_assert(kMsSlots - 1 <= kCodeSkip, "depth offsets within a window stay below the two special codes");
 constexpr uint32_t kMsTraceCap = 1u << 16;
 constexpr uint32_t kPartWords = 4;
 // Bottom-up LDS cache: the frontier words of the highest-degree vertices
@@ -88,7 +91,7 @@ struct MsDev {
     const uint32_t* sources;       // k
     const uint4* heavy;            // {v, first edge, end edge, degree}
     const uint32_t* perm;          // relabeled runs: perm[caller id] = traversal id (else null)
-    uint64_t* rec;                 // relabeled runs: n * W depth records (planes, unreached, any)
+    uint64_t* rec;                 // relabeled runs: n * W depth records (four planes of codes)
     unsigned long long* state;     // kMsState words: the loop state between launches of one run
     uint32_t* bar;                 // kBarWords
     uint32_t* abort;
@@ -619,6 +622,26 @@ __device__ __forceinline__ void ms_write_depths(uint32_t* out, size_t stride, ui
     }
 }
 
+// The same rows from a record's codes: base + code, INF for kCodeInf, no store
+// for kCodeSkip.
+__device__ __forceinline__ void ms_write_codes(uint32_t* out, size_t stride, uint32_t ns, uint32_t base, uint64_t q0,
+                                               uint64_t q1, uint64_t q2, uint64_t q3) {
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t sh = 32 * h;
+        const uint32_t p0 = (uint32_t)(q0 >> sh), p1 = (uint32_t)(q1 >> sh);
+        const uint32_t p2 = (uint32_t)(q2 >> sh), p3 = (uint32_t)(q3 >> sh);
+#pragma unroll
+        for (uint32_t j = 0; j < 32; ++j) {
+            if (sh + j >= ns) return;
+            const uint32_t c = ((p0 >> j) & 1u) | (((p1 >> j) & 1u) << 1) | (((p2 >> j) & 1u) << 2) |
+                               (((p3 >> j) & 1u) << 3);
+            const uint32_t val = c == kCodeInf ? GGPU_INF : base + c;
+            if (c != kCodeSkip) __builtin_nontemporal_store(val, out + (size_t)(sh + j) * stride);
+        }
+    }
+}
+
 typedef unsigned long long ms_u64x2 __attribute__((ext_vector_type(2)));
 
 template <int W>
@@ -651,13 +674,12 @@ __device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool
             }
             const uint64_t any = reached | unreached;
             if (D.perm) {
-                // the record of (v, w): plain 16-byte stores, released before the
-                // barrier that precedes their reads
+                // the record of (v, w), read by the next k_ms_unpermute: a pair
+                // with no plane bits gets kCodeInf if never reached, else kCodeSkip
+                const uint64_t hi = unreached | ~any;
                 ms_u64x2* r = (ms_u64x2*)(D.rec + ((size_t)v * W + w) * kMsRec);
-                r[0] = ms_u64x2{plane[0], plane[1]};
-                r[1] = ms_u64x2{plane[2], plane[3]};
-                r[2] = ms_u64x2{unreached, any};
-                r[3] = ms_u64x2{0ull, 0ull};  // whole 64-byte lines: no partial-line writes
+                r[0] = ms_u64x2{plane[0] | unreached, plane[1] | hi};
+                r[1] = ms_u64x2{plane[2] | hi, plane[3] | hi};
                 continue;
             }
             if (!__any(any != 0)) continue;
@@ -678,7 +700,7 @@ __device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool
 }
 
 // Relabeled runs, after the launch that retired a window: caller id u takes
-// the record of perm[u] (one random 48-byte read) and writes its depth rows
+// the record of perm[u] (one random 32-byte read) and writes its depth rows
 // with the wave's 64 consecutive u, as the direct depth pass does.
 template <int W>
 __global__ void __launch_bounds__(256) k_ms_unpermute(MsDev D) {
@@ -689,11 +711,13 @@ __global__ void __launch_bounds__(256) k_ms_unpermute(MsDev D) {
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             const ms_u64x2* r = (const ms_u64x2*)(D.rec + ((size_t)v * W + w) * kMsRec);
-            const ms_u64x2 a = r[0], b = r[1], c = r[2];
-            const uint64_t plane[kMsPlanes] = {a.x, a.y, b.x, b.y};
-            if (!__any(c.y != 0)) continue;
+            const ms_u64x2 a = r[0], b = r[1];
             const uint32_t s0 = (uint32_t)w * 64;
-            ms_write_depths(D.dist + (size_t)s0 * D.n + u, D.n, min(64u, D.k - s0), base, plane, c.x, c.y);
+            const uint32_t ns = min(64u, D.k - s0);
+            const uint64_t valid = ns == 64 ? ~0ull : (1ull << ns) - 1;
+            const uint64_t skip = ~a.x & a.y & b.x & b.y;  // kCodeSkip
+            if (!__any((skip & valid) != valid)) continue;
+            ms_write_codes(D.dist + (size_t)s0 * D.n + u, D.n, ns, base, a.x, a.y, b.x, b.y);
         }
     }
 }
