@@ -120,6 +120,14 @@ __device__ __forceinline__ uint64_t valid_word(const MsDev& D, int w) {
 __device__ __forceinline__ uint32_t gwave() { return (blockIdx.x * blockDim.x + threadIdx.x) / kWave; }
 __device__ __forceinline__ uint32_t nwaves() { return gridDim.x * blockDim.x / kWave; }
 
+// The frontier / mark word a lane owns in the round starting at r0: consecutive
+// waves take consecutive words and the lanes of a wave words nw apart, so a
+// run of busy words (the degree numbering packs the hubs, and the low-degree
+// tail, into a few) spreads over many waves. Words past nb read as empty.
+__device__ __forceinline__ uint32_t ms_word_of(uint64_t r0, uint32_t lane, uint32_t nw) {
+    return (uint32_t)(r0 + (uint64_t)lane * nw + gwave());
+}
+
 // Per-block counters of the level being produced, one write-through store each.
 __device__ void ms_flush(const MsDev& D, uint32_t slot, const MsAcc& a) {
     __shared__ unsigned long long s_red[kMsThreads / kWave];
@@ -432,11 +440,12 @@ __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex)
     const uint64_t* fb = D.fb + (size_t)cs * D.nb;
     uint64_t* fbn = D.fb + (size_t)ns * D.nb;
     const uint32_t lane = lane_id();
-    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t nw = nwaves();
 
     // light frontier vertices: each lane owns a frontier word and, round by
     // round, takes its next set bit, so 64 vertices of 64 words run side by side
-    for (uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x; wi - lane < D.nb; wi += stride) {
+    for (uint64_t r0 = 0; r0 < D.nb; r0 += (uint64_t)nw * kWave) {
+        const uint32_t wi = ms_word_of(r0, lane, nw);
         uint64_t fw = wi < D.nb ? fb[wi] : 0ull;
         while (__ballot(fw != 0)) {
             uint32_t v = 0xFFFFFFFFu;
@@ -498,46 +507,72 @@ __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex)
     }
 }
 
+// Settles marked vertex u: keeps next & ~visit as its new bits, clears next.
+// Returns the number of new (source, vertex) pairs.
+template <int W>
+__device__ __forceinline__ uint32_t ms_settle(const MsDev& D, uint64_t* Fn, uint32_t u, MsAcc& a) {
+    uint32_t pc = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const size_t i = (size_t)u * W + w;
+        const uint64_t vis = D.visit[i];
+        const uint64_t nw = D.next[i] & ~vis;
+        store_through(&D.next[i], 0ull);
+        if (nw) {
+            store_through(&Fn[i], nw);
+            store_through(&D.visit[i], vis | nw);
+            pc += (uint32_t)__popcll(nw);
+        }
+    }
+    if (pc) {
+        const uint64_t dg = D.rp[u + 1] - D.rp[u];
+        a.pairs += dg * pc;
+        a.edges += dg;
+    }
+    return pc;
+}
+
+// Marked words with more set bits than this are settled by the whole wave.
+constexpr uint32_t kMsDenseBits = 8;
+
 template <int W>
 __device__ void ms_td_collect(const MsDev& D, uint32_t level, MsAcc& a) {
     const uint32_t ns = (level + 1) % D.C;
     uint64_t* Fn = D.F + (size_t)ns * D.n * W;
     uint64_t* fbn = D.fb + (size_t)ns * D.nb;
     const uint32_t lane = lane_id();
-    const uint32_t stride = gridDim.x * blockDim.x;
-    // each lane owns one marked word and settles its vertices round by round
-    for (uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x; wi - lane < D.nb; wi += stride) {
+    const uint32_t nw = nwaves();
+    for (uint64_t r0 = 0; r0 < D.nb; r0 += (uint64_t)nw * kWave) {
+        const uint32_t wi = ms_word_of(r0, lane, nw);
         const uint64_t marked = wi < D.nb ? fbn[wi] : 0ull;
-        uint64_t mk = marked, found = 0;
+        // sparse words: each lane settles its word's vertices round by round
+        const bool dense = (uint32_t)__popcll(marked) > kMsDenseBits;
+        uint64_t mk = dense ? 0ull : marked, found = 0;
         while (__ballot(mk != 0)) {
             if (mk) {
                 const uint32_t j = (uint32_t)(__ffsll((long long)mk) - 1);
                 mk &= mk - 1;
-                const uint32_t u = wi * 64 + j;
-                uint32_t pc = 0;
-#pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    const size_t i = (size_t)u * W + w;
-                    const uint64_t vis = D.visit[i];
-                    const uint64_t nw = D.next[i] & ~vis;
-                    store_through(&D.next[i], 0ull);
-                    if (nw) {
-                        store_through(&Fn[i], nw);
-                        store_through(&D.visit[i], vis | nw);
-                        pc += (uint32_t)__popcll(nw);
-                    }
-                }
-                if (pc) {
-                    const uint64_t dg = D.rp[u + 1] - D.rp[u];
-                    a.pairs += dg * pc;
-                    a.edges += dg;
-                    found |= 1ull << j;
-                }
+                if (ms_settle<W>(D, Fn, wi * 64 + j, a)) found |= 1ull << j;
             }
         }
-        if (marked) {
+        if (marked && !dense) {
             store_through(&fbn[wi], found);
             a.size += (uint64_t)__popcll(found);
+        }
+        // dense words (in the degree numbering the hubs fill whole words): the
+        // wave takes them one at a time, a lane per vertex
+        uint64_t dw = __ballot(dense);
+        while (dw) {
+            const int j = __ffsll((long long)dw) - 1;
+            dw &= dw - 1;
+            const uint32_t wj = __shfl(wi, j, kWave);
+            const uint64_t word = __shfl(marked, j, kWave);
+            const bool f = ((word >> lane) & 1ull) && ms_settle<W>(D, Fn, wj * 64 + lane, a) != 0;
+            const uint64_t fm = __ballot(f);
+            if (lane == 0) {
+                store_through(&fbn[wj], fm);
+                a.size += (uint64_t)__popcll(fm);
+            }
         }
     }
 }
