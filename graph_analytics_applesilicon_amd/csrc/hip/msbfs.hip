@@ -53,7 +53,8 @@ constexpr uint32_t kMsRec = 4;          // u64 words per (vertex, source word) d
 // Codes of a depth record (the four plane bits of a source): 0..13 depth - base,
 // kCodeSkip: not reached in this window (an earlier window wrote it), kCodeInf: never reached.
 constexpr uint32_t kCodeSkip = 14, kCodeInf = 15;
-// Loop state of a relabeled run between launches (u64 words; zero before the first):
+// Loop state of a relabeled run between launches (u64 words; zero before the first
+// launch of a run: zeroed at allocation and by the run's last launch):
 enum : uint32_t {
     kStResume = 0,   // 1: resume at kStLevel, whose action kStAct was decided by the previous launch
     kStLevel,
@@ -63,7 +64,6 @@ enum : uint32_t {
     kStCounts,       // 3 words: levels run per action
     kStCumPairs = kStCounts + 3,
     kStBase,         // first level of the window whose records the un-permute reads
-    kStDone,         // 1: the traversal has ended
     kStLoopTicks,    // device clock spent in the level loop by earlier launches
     kStSetupTicks,
     kMsState
@@ -95,7 +95,7 @@ struct MsDev {
     unsigned long long* state;     // kMsState words: the loop state between launches of one run
     uint32_t* bar;                 // kBarWords
     uint32_t* abort;
-    uint32_t* out;                 // {levels, 0, actions[3], 0, clock at start, level 0, loop end, finish}
+    uint32_t* out;                 // {levels, done, actions[3], 0, setup, loop, finish clock ticks}
     ggpu_level* trace;             // trace_cap entries
     float* feat;                   // trace_cap x 5 policy features (as the policy sees them)
     const uint8_t* actions;        // scripted policy
@@ -891,7 +891,6 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
                         for (int i = 0; i < 3; ++i) st[kStCounts + i] = counts[i];
                         st[kStCumPairs] = cum_pairs;
                         st[kStBase] = base;
-                        st[kStDone] = 0;
                         st[kStLoopTicks] += wall_clock64() - clk[1];
                         if (!resume) st[kStSetupTicks] = clk[1] - clk[0];
                     }
@@ -929,6 +928,7 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         clk[3] = wall_clock64();
         D.out[0] = level;
+        D.out[1] = 1;  // the traversal has ended (a relabeled launch that retires a window leaves 0)
         D.out[2] = counts[0];
         D.out[3] = counts[1];
         D.out[4] = counts[2];
@@ -940,7 +940,8 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
                 D.out[7] += (uint32_t)D.state[kStLoopTicks];
             }
             D.state[kStBase] = base;
-            D.state[kStDone] = 1;
+            D.state[kStResume] = 0;  // the next run starts afresh
+            D.state[kStLoopTicks] = 0;
         }
     }
 }
@@ -1145,6 +1146,7 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             GGPU_HIP_TRY(hipMemsetAsync(S.next.p, 0, words * 8, s));
             GGPU_HIP_TRY(hipMemsetAsync(S.F.p, 0, (size_t)S.C * words * 8, s));
             GGPU_HIP_TRY(hipMemsetAsync(S.fb.p, 0, (size_t)S.C * nb * 8, s));
+            if (S.st.p) GGPU_HIP_TRY(hipMemsetAsync(S.st.p, 0, kMsState * 8, s));
             S.clean = true;
         }
         const uint32_t G = ms_grid(ctx, S, W);
@@ -1152,7 +1154,10 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         S.dist.alloc((size_t)k * n);
         if (relabel) {
             S.rec.alloc(words * kMsRec);
-            S.st.alloc(kMsState);
+            if (!S.st.p) {
+                S.st.alloc(kMsState);
+                GGPU_HIP_TRY(hipMemsetAsync(S.st.p, 0, kMsState * 8, s));
+            }
         }
         S.k = k;
         S.sources.alloc(k);
@@ -1165,10 +1170,11 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             GGPU_HIP_TRY(hipMemcpyAsync(S.actions.p, o.actions, o.n_actions, hipMemcpyHostToDevice, s));
         }
         const bool learned = o.policy == GGPU_POLICY_LEARNED && ctx->policy_set[GGPU_POLICY_BATCHED];
-        std::vector<uint32_t> src(sources, sources + k);
-        if (relabel)
-            for (uint32_t& x : src) x = S.perm_host[x];
-        GGPU_HIP_TRY(hipMemcpy(S.sources.p, src.data(), 4ull * k, hipMemcpyHostToDevice));
+        // pinned staging (the context's): sources (mapped to the traversal numbering)
+        // in the first KiB, the control words read back after each launch behind them
+        uint32_t* pin = (uint32_t*)ctx_pinned(ctx, 4096);
+        for (uint32_t i = 0; i < k; ++i) pin[i] = relabel ? S.perm_host[sources[i]] : sources[i];
+        GGPU_HIP_TRY(hipMemcpyAsync(S.sources.p, pin, 4ull * k, hipMemcpyHostToDevice, s));
         GGPU_HIP_TRY(hipMemsetAsync(S.ctl.p, 0, (kBarWords + 16) * sizeof(uint32_t), s));
 
         MsDev D{};
@@ -1214,13 +1220,12 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         void* args[] = {&D};
         S.have = false;
         S.clean = false;  // until the launch is known to have finished
-        if (relabel) GGPU_HIP_TRY(hipMemsetAsync(S.st.p, 0, kMsState * 8, s));
         GGPU_HIP_TRY(hipEventRecord(S.ev[0], s));
         // plain launches, one workgroup per CU (the LDS footprint allows no second):
         // the barrier is the kernel's own, bounded (see the single-source loop).
         // A relabeled run takes one launch per window of C levels, each followed
         // by the un-permute of that window's depth records.
-        uint32_t ctl[16];
+        uint32_t* ctl = pin + 256;
         for (uint32_t launch = 0;; ++launch) {
             if (launch) GGPU_HIP_TRY(hipMemsetAsync(S.ctl.p, 0, kBarWords * sizeof(uint32_t), s));
             GGPU_HIP_TRY(hipLaunchKernel(ms_kernel(W), dim3(G), dim3(kMsThreads), args, 0, s));
@@ -1228,13 +1233,10 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
                 GGPU_HIP_TRY(hipLaunchKernel(ms_unpermute_kernel(W), dim3(ms_unpermute_grid(ctx, n)), dim3(256),
                                              args, 0, s));
             GGPU_HIP_TRY(hipEventRecord(S.ev[1], s));  // the last launch's record is the one timed
-            unsigned long long done = 1;
-            if (relabel)
-                GGPU_HIP_TRY(hipMemcpyAsync(&done, S.st.p + kStDone, 8, hipMemcpyDeviceToHost, s));
-            GGPU_HIP_TRY(hipMemcpyAsync(ctl, S.ctl.p + kBarWords, sizeof ctl, hipMemcpyDeviceToHost, s));
+            GGPU_HIP_TRY(hipMemcpyAsync(ctl, S.ctl.p + kBarWords, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             GGPU_HIP_TRY(hipStreamSynchronize(s));
             if (ctl[0]) throw ArgError("batched BFS: grid barrier timed out", GGPU_ETIMEOUT);
-            if (done) break;
+            if (ctl[5]) break;  // out[1]: the traversal has ended
         }
 
         S.clean = true;
