@@ -12,7 +12,7 @@
 //   top-down: a frontier vertex ORs F_l[v] & ~visit[u] into next[u] of its
 //     neighbours; after a barrier each touched vertex keeps next & ~visit
 //     as its new bits, so each (source, vertex) pair is discovered once.
-// Vertices of degree >= kMsHeavy are scanned / pushed as kMsChunk-edge items
+// Vertices of degree >= heavy_min are scanned / pushed as chunk-edge items
 // by whole waves (both directions), the rest lane per vertex.
 // Depths are not written during the traversal: the level slots are turned
 // into depths through bit planes (plane p = OR of F over levels whose offset
@@ -45,8 +45,14 @@
 namespace ggpu {
 
 constexpr uint32_t kMsThreads = 1024;
-constexpr uint32_t kMsHeavy = 32;       // degree from which a vertex is handled as wave items
-constexpr uint32_t kMsChunk = 1024;     // edges per item
+// Degree from which a vertex is handled as wave items, and edges per item: in the
+// caller's numbering a wave's 64 consecutive vertices mix degrees, so vertices of
+// degree >= 32 go to wave items; in the degree numbering they are alike, and
+// lane-per-vertex scans pay up to degree 63 (64-source rmat22 batch: 1.72 ms at
+// 32 / 1024, 1.54 ms at 64 / 2048; caller numbering 2.55 ms at 32 / 1024, 2.84 ms
+// at 64 / 2048).
+constexpr uint32_t kMsHeavyGiven = 32, kMsChunkGiven = 1024;
+constexpr uint32_t kMsHeavyDegree = 64, kMsChunkDegree = 2048;
 constexpr uint32_t kMsSlots = 15;       // level slots: a window retires <= 14 levels (four depth planes)
 constexpr uint32_t kMsPlanes = 4;
 constexpr uint32_t kMsRec = 4;          // u64 words per (vertex, source word) depth record: the planes
@@ -106,6 +112,7 @@ struct MsDev {
     uint64_t valid_last;           // valid bits of the last source word
     uint32_t n, k, nb, n_heavy, C, trace_cap, n_actions, policy;
     uint32_t n_hot;                // <= kMsHotWords / W
+    uint32_t heavy_min;            // degree from which a vertex is in the heavy item list
 };
 
 struct MsAcc {
@@ -225,7 +232,7 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, MsAcc
         if (v < D.n) {
             const uint32_t b = D.rp[v], e = D.rp[v + 1];
             const uint32_t dg = e - b;
-            if (dg > 0 && dg < kMsHeavy) {
+            if (dg > 0 && dg < D.heavy_min) {
                 uint64_t need[W], acc[W];
                 bool any = false;
 #pragma unroll
@@ -457,7 +464,7 @@ __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex)
                 // binned: the lane pushes its vertex
                 if (v != 0xFFFFFFFFu) {
                     const uint32_t b = D.rp[v], e = D.rp[v + 1];
-                    if (e - b < kMsHeavy) {
+                    if (e - b < D.heavy_min) {
                         uint64_t f[W];
 #pragma unroll
                         for (int w = 0; w < W; ++w) f[w] = F[(size_t)v * W + w];
@@ -471,7 +478,7 @@ __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex)
                     b = D.rp[v];
                     e = D.rp[v + 1];
                 }
-                uint64_t todo = __ballot(v != 0xFFFFFFFFu && e - b < kMsHeavy);
+                uint64_t todo = __ballot(v != 0xFFFFFFFFu && e - b < D.heavy_min);
                 while (todo) {
                     const int j = __ffsll((long long)todo) - 1;
                     todo &= todo - 1;
@@ -1075,6 +1082,8 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             S.order = o.batch_order;
         }
         const Csr& hg = relabel ? S.relabeled : e.host;
+        const uint32_t heavy_min = relabel ? kMsHeavyDegree : kMsHeavyGiven;
+        const uint32_t chunk = relabel ? kMsChunkDegree : kMsChunkGiven;
         const uint32_t W = (k + 63) / 64;
         const uint32_t n = e.n;
         const uint32_t nb = (n + 63) / 64;
@@ -1087,9 +1096,9 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             std::vector<uint4> items;
             for (uint32_t v = 0; v < n; ++v) {
                 const uint32_t dg = hg.deg(v);
-                if (dg < kMsHeavy) continue;
-                for (uint32_t x = hg.rp[v]; x < hg.rp[v + 1]; x += kMsChunk)
-                    items.push_back(make_uint4(v, x, std::min(x + kMsChunk, hg.rp[v + 1]), dg));
+                if (dg < heavy_min) continue;
+                for (uint32_t x = hg.rp[v]; x < hg.rp[v + 1]; x += chunk)
+                    items.push_back(make_uint4(v, x, std::min(x + chunk, hg.rp[v + 1]), dg));
             }
             S.n_heavy = (uint32_t)items.size();
             S.heavy.alloc(items.size() + 1);
@@ -1186,6 +1195,7 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         D.cib = S.cib.p;
         D.hot = S.hot.p;
         D.n_hot = S.n_hot;
+        D.heavy_min = heavy_min;
         D.visit = S.visit.p;
         D.next = S.next.p;
         D.F = S.F.p;
