@@ -37,7 +37,8 @@ BFS_VARIANTS = {"gpu_edge": BFS_EDGE, "gpu_binned": BFS_BINNED, "gpu_do": BFS_DO
                 "gpu_vertex": BFS_VERTEX, "gpu_naive": BFS_NAIVE}
 LOOP_HOST, LOOP_GRAPH, LOOP_PERSISTENT = 0, 1, 2
 POLICY_HEURISTIC, POLICY_LEARNED, POLICY_SCRIPTED = 0, 1, 2
-BATCH_ORDER_GIVEN, BATCH_ORDER_DEGREE = 0, 1  # batched BFS traversal numbering (depths in caller ids)
+# batched BFS traversal numbering (depths in caller ids); AUTO: DEGREE when max degree >= 8x mean
+BATCH_ORDER_GIVEN, BATCH_ORDER_DEGREE, BATCH_ORDER_AUTO = 0, 1, 2
 ACTION_TD_EDGE, ACTION_TD_BINNED, ACTION_BU = 0, 1, 2
 PR_SCALAR, PR_VECTOR, PR_ADAPTIVE, PR_GS, PR_PUSH = 1, 2, 3, 4, 5
 PR_LAYOUT_STREAM, PR_LAYOUT_BINS, PR_LAYOUT_HOT = 0, 1, 2

@@ -530,7 +530,7 @@ void ggpu_bfs_opts_default(ggpu_bfs_opts* o) {
     o->policy = GGPU_POLICY_HEURISTIC;
     o->graph_levels = 8;
     o->time_levels = 1;
-    o->batch_order = GGPU_BATCH_ORDER_DEGREE;
+    o->batch_order = GGPU_BATCH_ORDER_AUTO;
 }
 
 int ggpu_bfs_parse_variant(const char* name, int* variant) {

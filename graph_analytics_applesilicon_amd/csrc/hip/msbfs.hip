@@ -1054,12 +1054,23 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         if (o.policy == GGPU_POLICY_SCRIPTED && (!o.actions || !o.n_actions))
             throw ArgError("scripted policy needs actions");
         if (!(o.alpha > 0) || !(o.beta > 0)) throw ArgError("alpha and beta must be positive");
-        if (o.batch_order > GGPU_BATCH_ORDER_DEGREE) throw ArgError("unknown batch_order");
+        if (o.batch_order > GGPU_BATCH_ORDER_AUTO) throw ArgError("unknown batch_order");
         GGPU_HIP_TRY(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
         MsState& S = ms_state(e);
-        const bool relabel = o.batch_order == GGPU_BATCH_ORDER_DEGREE && e.n > 0;
-        if (S.order != o.batch_order) {
+        uint32_t order = o.batch_order;
+        if (order == GGPU_BATCH_ORDER_AUTO) {
+            // skewed degrees: the renumbering packs the most-gathered frontier words
+            // into few lines; on near-regular graphs (grids, paths) it only adds the
+            // un-permute and, on deep traversals, a relaunch per window of levels
+            // (64-source batch on the 2932-level grid_c3: 178 ms given, 188 ms degree)
+            uint32_t dmax = 0;
+            for (uint32_t v = 0; v < e.n; ++v) dmax = std::max(dmax, e.host.deg(v));
+            order = e.n && (double)dmax >= 8.0 * (double)e.m / (double)e.n ? GGPU_BATCH_ORDER_DEGREE
+                                                                            : GGPU_BATCH_ORDER_GIVEN;
+        }
+        const bool relabel = order == GGPU_BATCH_ORDER_DEGREE && e.n > 0;
+        if (S.order != order) {
             S.heavy_ready = false;
             S.hot_w = 0;
             S.relabeled = Csr();
@@ -1079,7 +1090,7 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
                     GGPU_HIP_TRY(hipMemcpy(S.ci2.p, S.relabeled.ci.data(), S.relabeled.ci.size() * 4, hipMemcpyHostToDevice));
                 GGPU_HIP_TRY(hipMemcpy(S.perm.p, S.perm_host.data(), (size_t)e.n * 4, hipMemcpyHostToDevice));
             }
-            S.order = o.batch_order;
+            S.order = order;
         }
         const Csr& hg = relabel ? S.relabeled : e.host;
         const uint32_t heavy_min = relabel ? kMsHeavyDegree : kMsHeavyGiven;

@@ -66,6 +66,7 @@ int ggpu_ctx_sync(ggpu_ctx* ctx);
  * depths always come back in the caller's ids). */
 #define GGPU_BATCH_ORDER_GIVEN 0   /* the caller's numbering */
 #define GGPU_BATCH_ORDER_DEGREE 1  /* renumbered by descending degree (reorder "degree") on the first run */
+#define GGPU_BATCH_ORDER_AUTO 2    /* DEGREE when the maximum degree is >= 8x the mean, else GIVEN */
 
 typedef struct ggpu_bfs_opts {
     uint32_t tg;         /* workgroup size of the level kernels: multiple of 64 in [64, 1024] */
@@ -82,7 +83,7 @@ typedef struct ggpu_bfs_opts {
     uint32_t n_actions;
     uint32_t graph_levels;  /* GGPU_LOOP_GRAPH: levels captured per graph launch */
     uint32_t time_levels;   /* HOST loop: 1 = record per-level GPU time with events */
-    uint32_t batch_order;   /* batched BFS: GGPU_BATCH_ORDER_* (default DEGREE) */
+    uint32_t batch_order;   /* batched BFS: GGPU_BATCH_ORDER_* (default AUTO) */
 } ggpu_bfs_opts;
 void ggpu_bfs_opts_default(ggpu_bfs_opts* o);
 /* "gpu_edge" | "gpu_binned" | "gpu_do" | "hybrid" | "gpu_vertex" | "gpu_naive" -> GGPU_BFS_*

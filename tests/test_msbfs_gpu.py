@@ -55,9 +55,9 @@ MODES = {
     "all-bottom-up": dict(alpha=1e12, beta=1e12),
     "all-top-down": dict(alpha=1e-9),
 }
-# traversal numbering: renumbered by degree (the default; depths are mapped
-# back to the caller's ids) and the caller's own
-ORDERS = {"degree": G.BATCH_ORDER_DEGREE, "given": G.BATCH_ORDER_GIVEN}
+# traversal numbering: renumbered by degree (depths are mapped back to the
+# caller's ids), the caller's own, and the default choice between them
+ORDERS = {"degree": G.BATCH_ORDER_DEGREE, "given": G.BATCH_ORDER_GIVEN, "auto": G.BATCH_ORDER_AUTO}
 
 
 @pytest.mark.parametrize("order", list(ORDERS))
@@ -79,7 +79,7 @@ def test_batched_scripted_actions(name):
         src = _batch(g, k, seed=k)
         for trial in range(4):
             actions = rng.integers(0, 3, size=int(rng.integers(1, 9))).tolist()
-            order = list(ORDERS)[trial % 2]
+            order = list(ORDERS)[trial % 3]
             res, trace = _check(g, bfs, src, G.bfs_options(policy=G.POLICY_SCRIPTED, actions=actions,
                                                            batch_order=ORDERS[order]),
                                 f"{name} k={k} actions={actions} {order}")
@@ -99,7 +99,7 @@ def test_batched_all_same_source_and_state_reuse():
     d2, _, _ = bfs.run_batch(src)
     assert np.array_equal(d1, d2)
     # switching the numbering back and forth on one object rebuilds its arrays
-    for order in ("given", "degree", "given", "degree"):
+    for order in ("given", "degree", "auto", "given", "degree"):
         d3, _, _ = bfs.run_batch(src, G.bfs_options(batch_order=ORDERS[order]))
         assert np.array_equal(d1, d3), order
 
@@ -130,7 +130,7 @@ def test_batched_errors():
     with pytest.raises(ValueError):
         fresh.run_batch(np.array([0], dtype=np.uint32), G.bfs_options(policy=G.POLICY_SCRIPTED))
     with pytest.raises(ValueError):
-        fresh.run_batch(np.array([0], dtype=np.uint32), G.bfs_options(batch_order=2))
+        fresh.run_batch(np.array([0], dtype=np.uint32), G.bfs_options(batch_order=3))
     # nothing has run on `fresh`: fetching is a state error
     out = np.zeros(g.n, dtype=np.uint32)
     rc = fresh.ctx.lib.ggpu_msbfs_fetch_dist(fresh._h, out.ctypes.data_as(G.C.POINTER(G.C.c_uint32)))
