@@ -252,7 +252,7 @@ def bfs_part(G, O, args, ctx):
         bfs.run_batch(sources, given, fetch=False)
         out["given_order_batched_ms"] = float(np.median([bfs.run_batch(sources, given, fetch=False)[1].gpu_ms
                                                          for _ in range(3)]))
-        # S = 256 sources in one launch (the second batch size of the workload)
+        # S = 256 sources in one batch (the second batch size of the workload)
         s256 = g.pick_sources(256, 2)
         bfs.run_batch(s256, bopts, fetch=False)
         ms256 = float(np.median([bfs.run_batch(s256, bopts, fetch=False)[1].gpu_ms for _ in range(3)]))

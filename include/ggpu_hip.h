@@ -113,11 +113,12 @@ uint32_t ggpu_bfs_last_levels(const ggpu_bfs* b);
 typedef struct ggpu_msbfs_result {
     uint32_t levels;          /* levels of the batch (max over sources + 1) */
     uint32_t pad;
-    double gpu_ms;            /* device time of the whole batch (events) */
+    double gpu_ms;            /* device time of the whole batch (events around every launch of it) */
     uint64_t level_actions[3];/* levels run per action */
     double setup_ms;          /* device clock: launch start -> level 0 ready */
-    double levels_ms;         /* device clock: the level loop */
-    double finish_ms;         /* device clock: depths written, state cleared (block 0's view) */
+    double levels_ms;         /* device clock: the level loop (summed over a renumbered run's launches) */
+    double finish_ms;         /* device clock: depth pass and state cleared (block 0's view); a
+                                 renumbered run's un-permute kernel is in gpu_ms only */
 } ggpu_msbfs_result;
 /* opt->policy picks the per-level action (HEURISTIC: GAP rule on batch counts;
  * LEARNED: the MLP; SCRIPTED: opt->actions). */
