@@ -13,6 +13,7 @@
 //          [--tg 256] [--small-max 16 --large-min 256] [--inplace] [--ref-tol T] [--gs-pairs 2]
 //          [--threads N] [--warmup 2] [--trials 5] [--validate] [--csv out.csv] [--dump-ranks out.f64]
 //   gbench --graph X --algo msbfs [--batch 64] [--sources 64] [--policy heuristic|learned]
+//          [--batch-order auto|degree|given]
 //          [--policy-file W.f32] [--warmup 2] [--trials 5] [--validate] [--csv out.csv]
 //   every algorithm: [--ordering original|random|degree|rcm] [--order-seed 1] [--locality-csv loc.csv]
 //          (vertices relabeled before the run, untimed; results mapped back for validation)
@@ -512,6 +513,14 @@ int run_bfs(const Args& a, const Graph& stored, const std::string& dataset) {
 
 // ------------------------------------------------------- batched BFS (new)
 int run_msbfs(const Args& a, const Graph& stored, const std::string& dataset) {
+    const std::string border = a.get("batch-order", "auto");
+    uint32_t batch_order = GGPU_BATCH_ORDER_AUTO;
+    if (border == "degree") batch_order = GGPU_BATCH_ORDER_DEGREE;
+    else if (border == "given") batch_order = GGPU_BATCH_ORDER_GIVEN;
+    else if (border != "auto") {
+        std::fprintf(stderr, "unknown --batch-order '%s'. Available: auto degree given\n", border.c_str());
+        return 2;
+    }
     Graph g = symmetrized(stored);
     const std::string ordering = a.get("ordering", "original");
     const std::vector<uint32_t> perm = ordering_perm(a, g, dataset);
@@ -528,6 +537,7 @@ int run_msbfs(const Args& a, const Graph& stored, const std::string& dataset) {
     o.alpha = a.real("alpha", 15.0);
     o.beta = a.real("beta", 18.0);
     set_policy(a, dev, o, GGPU_POLICY_BATCHED);
+    o.batch_order = batch_order;
     ggpu_bfs* gb = nullptr;
     dev.check(ggpu_bfs_create(dev.ctx, g.n(), g.m(), ggpu_graph_row_ptr(g.g), ggpu_graph_col_idx(g.g), &gb), "upload");
     std::unique_ptr<ggpu_bfs, void (*)(ggpu_bfs*)> owner(gb, ggpu_bfs_destroy);
@@ -546,7 +556,8 @@ int run_msbfs(const Args& a, const Graph& stored, const std::string& dataset) {
     const uint32_t batch = (uint32_t)std::min<long long>(256, std::max<long long>(1, a.num("batch", 64)));
     const int warmup = (int)a.num("warmup", 2), trials = (int)a.num("trials", 5);
     const bool validate = a.has("validate");
-    const std::string params = "batch=" + std::to_string(batch) + " policy=" + a.get("policy", "heuristic");
+    const std::string params =
+        "batch=" + std::to_string(batch) + " policy=" + a.get("policy", "heuristic") + " batch_order=" + border;
     std::printf("%s: n %llu, symmetrized m %llu\nmsbfs [%s], %zu sources, %d warm-up + %d trial(s)%s\n",
                 dataset.c_str(), (unsigned long long)g.n(), (unsigned long long)g.m(), params.c_str(), sources.size(),
                 warmup, trials, validate ? ", validating" : "");

@@ -33,6 +33,7 @@ def test_exit_codes():
     assert gbench("--graph", "synth:rmat12", "--algo", "bfs", "--variant", "gpu_nope").returncode == 2
     assert gbench("--graph", "synth:rmat12", "--algo", "pr", "--variant", "gpu_nope").returncode == 2
     assert gbench("--graph", "synth:rmat12", "--algo", "sssp").returncode == 2
+    assert gbench("--graph", "synth:rmat12", "--algo", "msbfs", "--batch-order", "bogus").returncode == 2
     assert gbench("--graph", "synth:rmat12", "--algo", "bfs", "--source", "4096").returncode == 2
     assert gbench("--graph", "synth:rmat12", "--algo", "bfs", "--ordering", "spectral").returncode == 1
 
@@ -73,8 +74,10 @@ def test_gpu_variants_validate_under_every_ordering(tmp_path, ordering):
             r = gbench("--graph", graph, "--algo", "bfs", "--variant", "gpu_do", "--loop", loop, "--sources", "3",
                        *common)
             assert r.returncode == 0 and "validated OK" in r.stdout, (graph, loop, r.stdout + r.stderr)
-        r = gbench("--graph", graph, "--algo", "msbfs", "--sources", "70", "--batch", "64", *common)
-        assert r.returncode == 0 and "validated OK" in r.stdout, (graph, "msbfs", r.stdout + r.stderr)
+        for border in ("auto", "degree", "given"):
+            r = gbench("--graph", graph, "--algo", "msbfs", "--sources", "70", "--batch", "64", "--batch-order",
+                       border, *common)
+            assert r.returncode == 0 and "validated OK" in r.stdout, (graph, "msbfs", border, r.stdout + r.stderr)
     for graph in ("synth:rmat12_directed", "synth:star100k"):
         for variant in ("gpu_push", "gpu_pull_scalar", "gpu_pull_vector", "gpu_pull_adaptive"):
             r = gbench("--graph", graph, "--algo", "pr", "--variant", variant, *common)
