@@ -965,6 +965,7 @@ struct MsState {
     // graph (host copy for rebuilding them, device copy to traverse), perm[caller
     // id] = traversal id, and the depth records
     uint32_t order = 0xFFFFFFFFu;
+    uint32_t auto_order = 0xFFFFFFFFu;  // what AUTO resolves to on this graph (computed once)
     Csr relabeled;
     std::vector<uint32_t> perm_host;
     DBuf<uint32_t> rp2, ci2, perm;
@@ -1064,10 +1065,13 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             // into few lines; on near-regular graphs (grids, paths) it only adds the
             // un-permute and, on deep traversals, a relaunch per window of levels
             // (64-source batch on the 2932-level grid_c3: 178 ms given, 188 ms degree)
-            uint32_t dmax = 0;
-            for (uint32_t v = 0; v < e.n; ++v) dmax = std::max(dmax, e.host.deg(v));
-            order = e.n && (double)dmax >= 8.0 * (double)e.m / (double)e.n ? GGPU_BATCH_ORDER_DEGREE
-                                                                            : GGPU_BATCH_ORDER_GIVEN;
+            if (S.auto_order == 0xFFFFFFFFu) {
+                uint32_t dmax = 0;
+                for (uint32_t v = 0; v < e.n; ++v) dmax = std::max(dmax, e.host.deg(v));
+                S.auto_order = e.n && (double)dmax >= 8.0 * (double)e.m / (double)e.n ? GGPU_BATCH_ORDER_DEGREE
+                                                                                      : GGPU_BATCH_ORDER_GIVEN;
+            }
+            order = S.auto_order;
         }
         const bool relabel = order == GGPU_BATCH_ORDER_DEGREE && e.n > 0;
         if (S.order != order) {
