@@ -208,7 +208,11 @@ __device__ void ms_seed(const MsDev& D, MsAcc& a) {
 
 // ---------------------------------------------------------------- bottom-up
 template <int W>
-__device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, MsAcc& a) {
+// direct: gather the frontier words of every neighbour without testing the
+// frontier bitmap first (a word is zero outside the frontier); pays when most
+// arcs lead into the frontier, as the test then filters little and adds a
+// dependent gather.
+__device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, bool direct, MsAcc& a) {
     const uint32_t cs = level % D.C, ns = (level + 1) % D.C;
     const uint64_t* F = D.F + (size_t)cs * D.n * W;
     const uint64_t* fb = D.fb + (size_t)cs * D.nb;
@@ -250,7 +254,8 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, MsAcc
                         for (uint32_t i = 0; i < 4; ++i) u[i] = i < c ? cx[x + i] : 0u;
 #pragma unroll
                         for (uint32_t i = 0; i < 4; ++i)
-                            f[i] = i < c && !(u[i] & kMsHotBit) ? fb[u[i] >> 6] >> (u[i] & 63) : 0ull;
+                            f[i] = i < c && !(u[i] & kMsHotBit) ? (direct ? 1ull : fb[u[i] >> 6] >> (u[i] & 63))
+                                                                : 0ull;
 #pragma unroll
                         for (uint32_t i = 0; i < 4; ++i) {
                             if (i < c && (u[i] & kMsHotBit)) {
@@ -337,7 +342,7 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, MsAcc
                 }
 #pragma unroll
                 for (uint32_t i = 0; i < 4; ++i)
-                    f[i] = in[i] && !(u[i] & kMsHotBit) ? fb[u[i] >> 6] >> (u[i] & 63) : 0ull;
+                    f[i] = in[i] && !(u[i] & kMsHotBit) ? (direct ? 1ull : fb[u[i] >> 6] >> (u[i] & 63)) : 0ull;
 #pragma unroll
                 for (uint32_t i = 0; i < 4; ++i) {
                     if (in[i] && (u[i] & kMsHotBit)) {
@@ -913,7 +918,8 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
         if (act == GGPU_ACTION_BU) {
             // the hot copy (~25 us per level on 256 workgroups) pays while at least
             // 1/32 of the pair-edges are unreached
-            ms_bottom_up<W>(D, level, pairs_u * 32 > total_pairs, a);
+            // frontier edges: arcs into the frontier (the graph is symmetric)
+            ms_bottom_up<W>(D, level, pairs_u * 32 > total_pairs, (uint64_t)t.edges * 2 > D.m, a);
         } else {
             // a big push marks nothing and the collect scans every vertex (fewer
             // memory operations than one contended bitmap atomic per push)
