@@ -220,7 +220,13 @@ def bfs_part(G, O, args, ctx):
             bfs.run("gpu_do", int(s), plain_opts, trace=False, fetch=False)
         return 0.0
 
-    for _ in range(args.warmup):
+    # the first batched call also builds the engine's derived arrays (the degree-renumbered
+    # copy of the graph, heavy items, LDS-slot col_idx): reported, not timed as a step
+    t = time.perf_counter()
+    step()
+    ctx.sync()
+    out["first_call_ms"] = (time.perf_counter() - t) * 1e3
+    for _ in range(max(0, args.warmup - 1)):
         step()
     ctx.sync()
     t = time.perf_counter()
