@@ -98,7 +98,11 @@ def test_rmat22_c2_gpu_do_batch_and_gs():
     dist, res, _ = bfs.run_batch(src, G.bfs_options(policy=G.POLICY_LEARNED))
     for i, s in enumerate(checked):
         assert np.array_equal(dist[i], refs[s]), (i, s)
-    del dist
+    # all 64 rows: the default run (degree numbering, depths un-permuted) equals the
+    # batch traversed in the caller's numbering, which has neither step
+    given, _, _ = bfs.run_batch(src, G.bfs_options(policy=G.POLICY_LEARNED, batch_order=G.BATCH_ORDER_GIVEN))
+    assert np.array_equal(dist, given)
+    del dist, given
     od = g.degrees()
     conv = _pr64(rp, ci, od, 1000, 1e-10)
     jac = _pr64(rp, ci, od, 1000, 1e-6)
