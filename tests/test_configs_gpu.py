@@ -8,8 +8,10 @@
   gpu_do depths and per-level (direction, frontier) through the host,
   persistent and graph loops and hybrid; gpu_pull_scalar for 20 iterations.
 * rmat22_c2 (scale 22, ef 8, permuted): single-source gpu_do, the 64-source
-  batch under the learned policy (depths of the checked sources bit for bit),
-  gpu_pull_gs to tol 1e-6 judged against the converged FP64 ranks.
+  batch under the learned policy (six sources against the oracle; all 64 rows
+  equal to the caller-numbering batch and, like all 256 of a 256-source
+  batch, to each source's own sequential gpu_do run), gpu_pull_gs to tol 1e-6
+  judged against the converged FP64 ranks.
 * rmat23_c5 (scale 23, ef 16, a = .60): Jacobi certified to L1 < 1e-4; the
   64-seed personalized-PageRank block: every column keeps mass 1, two work
   assignments (given numbering, no chunks / degree numbering, 1024-entry
@@ -102,7 +104,20 @@ def test_rmat22_c2_gpu_do_batch_and_gs():
     # batch traversed in the caller's numbering, which has neither step
     given, _, _ = bfs.run_batch(src, G.bfs_options(policy=G.POLICY_LEARNED, batch_order=G.BATCH_ORDER_GIVEN))
     assert np.array_equal(dist, given)
-    del dist, given
+    del given
+    # every source of the 64 and of a 256-source batch bit-exact against its own
+    # sequential heuristic gpu_do run
+    seq = {}
+    for i, s in enumerate(src):
+        seq[int(s)] = bfs.run("gpu_do", int(s), trace=False).dist
+        assert np.array_equal(dist[i], seq[int(s)]), (i, int(s))
+    del dist
+    src256 = g.pick_sources(256, 2)
+    dist256, _, _ = bfs.run_batch(src256, G.bfs_options(policy=G.POLICY_LEARNED))
+    for i, s in enumerate(src256):
+        want = seq[int(s)] if int(s) in seq else bfs.run("gpu_do", int(s), trace=False).dist
+        assert np.array_equal(dist256[i], want), (i, int(s))
+    del dist256, seq
     od = g.degrees()
     conv = _pr64(rp, ci, od, 1000, 1e-10)
     jac = _pr64(rp, ci, od, 1000, 1e-6)
