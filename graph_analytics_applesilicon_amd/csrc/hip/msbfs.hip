@@ -1,6 +1,7 @@
 // Batched multi-source BFS (no reference counterpart): k <= 256 sources
 // traverse together in ONE persistent launch (a workgroup per CU), one grid barrier
-// per level.
+// per level (a renumbered run below adds an un-permute kernel, and a relaunch
+// per window of levels on deep graphs).
 //
 // State, W = ceil(k/64) u64 words per vertex, bit s%64 of word s/64 = source s:
 //   visit[v][w]       sources that have reached v
@@ -8,10 +9,12 @@
 //   fb_l[v/64]        bit v%64: F_l[v] != 0 (the level-l frontier)
 // Level l reads F_l / fb_l and writes F_{l+1} / fb_{l+1} / visit:
 //   bottom-up: an unfinished vertex ORs the F_l words of its frontier
-//     neighbours until every bit it still needs is present;
-//   top-down: a frontier vertex ORs F_l[v] & ~visit[u] into next[u] of its
-//     neighbours; after a barrier each touched vertex keeps next & ~visit
-//     as its new bits, so each (source, vertex) pair is discovered once.
+//     neighbours until every bit it still needs is present (testing fb_l
+//     first, unless most arcs lead into the frontier);
+//   top-down: a frontier vertex ORs F_l[v] (& ~visit[u] once most pair-edges
+//     are reached) into next[u] of its neighbours; after a barrier each touched
+//     vertex keeps next & ~visit as its new bits, so each (source, vertex)
+//     pair is discovered once.
 // Vertices of degree >= heavy_min are scanned / pushed as chunk-edge items
 // by whole waves (both directions), the rest lane per vertex.
 // Depths are not written during the traversal: the level slots are turned
@@ -19,7 +22,8 @@
 // has bit p) and written source-major with coalesced stores, when the C
 // slots are used up and at the end. That pass also clears what it read, so
 // visit / next / F / fb are all zero between launches.
-// A relabeled run traverses the graph renumbered by descending degree (the
+// A relabeled run (batch_order DEGREE, or AUTO on a graph whose maximum degree is
+// at least 8x the mean) traverses the graph renumbered by descending degree (the
 // frontier words most arcs gather then share a few MB of lines, which stay in
 // each XCD's L2): the depth pass writes a 32-byte record per vertex (four
 // planes, a 4-bit code per source) in that numbering, and a separate kernel, k_ms_unpermute,
