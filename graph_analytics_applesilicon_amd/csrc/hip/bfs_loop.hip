@@ -1,11 +1,14 @@
 // Device-driven single-source BFS level loops (no reference counterpart; the
 // reference syncs the host every level, src/metal/bfs_gpu.mm:83-104):
-//   GGPU_LOOP_PERSISTENT  one persistent launch runs every level; levels are
-//                         separated by the grid barrier of grid_sync.hpp;
+//   GGPU_LOOP_PERSISTENT  one persistent launch runs every level. Workgroup 0
+//                         alone folds the counters and decides each level; it
+//                         runs small top-down levels by itself and publishes
+//                         the others as commands the grid polls
+//                         (dl_protocol.hpp);
 //   GGPU_LOOP_GRAPH       a captured hipGraph of K levels (a decide kernel and a
 //                         level kernel per level) is replayed until the
 //                         frontier is empty; the host checks once per K levels.
-// Both make exactly the host loop's decisions: every block (persistent) or the
+// Both make exactly the host loop's decisions: workgroup 0 (persistent) or the
 // decide kernel (graph) runs decide_action / fold_level of bfs_engine.hpp on
 // the same per-level counters, so the trace (frontier, direction) equals the
 // host loop's and the oracle's.
@@ -18,10 +21,11 @@
 //   bottom-up: each unvisited vertex scans its neighbours for a frontier bit.
 // Vertices of degree >= kHeavyMin are handled as kHeavyChunk-edge wave items
 // in both directions (the engine's heavy list), the rest lane per vertex.
-// Writes read by other blocks after a barrier are atomics or write-through
-// stores; the persistent loop's barrier invalidates only L1, so data other
-// blocks wrote (counters, frontier and visited words) is read with coherent
-// loads while the read-only CSR stays cached (grid_sync.hpp).
+// In the persistent loop every datum another workgroup reads during the
+// launch (counters, frontier and visited words, the command word) is written
+// with device-scope atomics or write-through stores and read with coherent
+// loads; the read-only CSR stays cached. dist is only written with
+// write-through stores (the host reads it after the launch).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +33,7 @@
 #include <cstring>
 
 #include "bfs_engine.hpp"
+#include "dl_protocol.hpp"
 #include "grid_sync.hpp"
 
 namespace ggpu {
@@ -43,7 +48,6 @@ struct DlState {
     LoopState st;
     PolicyView pv;
     uint32_t started, prev_action, levels, done;
-    uint32_t sb_more, sb_pad;  // persistent: the single-workgroup episode ended with the traversal unfinished
     uint64_t t_prev;
     double tick_ms;
 };
@@ -58,9 +62,9 @@ struct DlDev {
     const uint4* heavy;             // {v, first edge, end edge, degree}, heaviest first
     DlState* ds;
     ggpu_level* trace;              // null: no trace
-    uint32_t* bar;                  // persistent barrier words, then abort
-    uint32_t* abort;
-    uint32_t* sb_flag;              // persistent: single-workgroup episodes finished (monotonic)
+    uint32_t* bar;                  // persistent: arrival counters (grid_sync.hpp layout)
+    uint32_t* abort;                // persistent: a workgroup gave up waiting
+    unsigned long long* cmd;        // persistent: workgroup 0's command word (dl_protocol.hpp)
     uint32_t n_heavy, n, nb, trace_cap;
     uint64_t m;
 };
@@ -333,20 +337,19 @@ __device__ bool dl_decide(const DlDev& D, DlBlock& B, bool started, const unsign
 // ------------------------------------------------- single-workgroup levels
 // A level whose frontier is small costs the full-grid loop a chain of
 // cross-XCD round trips (coherent counter reads, device-scope atomics, the
-// barrier) that does not shrink with the frontier. While the decision is top-
-// down and the frontier has at most kSbMax vertices and kSbEdges edges, the
-// persistent kernel runs levels on workgroup 0 alone: frontier queues in LDS,
-// visited and frontier bits updated with workgroup-scope atomics in that
-// XCD's L2, no grid barrier; the other workgroups wait on a flag. A level is
-// edge-parallel: the frontier's row offsets are loaded once (their sum is
-// also the previous level's next_edges) and scanned into an edge -> vertex
-// map in LDS, then every edge's col_idx load and visited atomic is issued at
-// once, so a level is about three memory round trips. The decisions come from the same fold / decide code with the same
-// counters (found, next_edges), so the trace equals the other loops'; the
-// frontier bitmaps are kept exactly as the grid loop keeps them, so either
-// loop can take over at any level.
-constexpr uint32_t kSbMax = 4096;
-constexpr uint32_t kSbEdges = 1u << 14;
+// arrival wait) that does not shrink with the frontier. While the decision is
+// top-down and the frontier has at most kSbMax vertices and kSbEdges edges
+// (sb_ok, dl_protocol.hpp), workgroup 0 runs levels alone: frontier queues in
+// LDS, no arrivals; the other workgroups keep polling the command word. A
+// level is edge-parallel: the frontier's row offsets are loaded once (their
+// sum is also the previous level's next_edges) and scanned into an edge ->
+// vertex map in LDS, then every edge's col_idx load and visited atomic is
+// issued at once, so a level is about three memory round trips. The decisions
+// come from the same fold / decide code with the same counters (found,
+// next_edges), so the trace equals the other loops'; the frontier bitmaps are
+// kept exactly as the grid levels keep them (read with coherent loads, bits
+// set by device-scope atomics, slots cleared by write-through stores), so
+// either side can take over at any level.
 constexpr uint32_t kSbBatch = 8;  // edges per lane in flight (kSbEdges / blockDim rounds at most 2)
 
 struct SbShared {
@@ -357,10 +360,6 @@ struct SbShared {
     uint32_t qn[3];           // queue counts (may exceed kSbMax: then the queue is cut)
     uint64_t wsum[2][kDlThreads / kWave];
 };
-
-__device__ __forceinline__ bool sb_ok(const LoopState& st) {
-    return st.action != GGPU_ACTION_BU && st.size <= kSbMax && st.total <= kSbEdges;
-}
 
 // Exclusive prefix over the block (1024 threads) of a u64 per thread, and
 // the block total of a second u64; returns the first's total.
@@ -402,8 +401,8 @@ __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const floa
         const uint64_t* fb = D.fb + (size_t)(B.st.level % 3) * D.nb;
         uint64_t* fold = D.fb + (size_t)((B.st.level + 2) % 3) * D.nb;
         for (uint32_t i = tid; i < D.nb; i += blockDim.x) {
-            uint64_t w = fb[i];
-            fold[i] = 0ull;
+            uint64_t w = load_coherent(&fb[i]);
+            store_through(&fold[i], 0ull);
             while (w) {
                 const uint32_t b = (uint32_t)__ffsll((long long)w) - 1;
                 w &= w - 1;
@@ -457,9 +456,10 @@ __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const floa
         }
         cut_edges = 0;
         const uint32_t level = B.st.level, depth = level + 1;
+        if (tid == 0) store_through(D.cmd, dl_cmd(kCmdNone, level, B.st.action));  // heartbeat
         uint64_t* fbn = D.fb + (size_t)((level + 1) % 3) * D.nb;
         uint64_t* fold = D.fb + (size_t)((level + 2) % 3) * D.nb;
-        for (uint32_t i = tid; i < pn; i += blockDim.x) fold[S.q[prv][i] >> 6] = 0ull;
+        for (uint32_t i = tid; i < pn; i += blockDim.x) store_through(&fold[S.q[prv][i] >> 6], 0ull);
         // every thread's clears (and its previous level's stores and atomics) land
         // before any of this level's atomics on the slot just cleared
         __builtin_amdgcn_s_waitcnt(0);
@@ -483,15 +483,15 @@ __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const floa
             for (uint32_t k = 0; k < kSbBatch; ++k)
                 old[k] = e0 + k * blockDim.x < E
                              ? __hip_atomic_fetch_or(&D.vis[u[k] >> 6], 1ull << (u[k] & 63), __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_WORKGROUP)
+                                                     __HIP_MEMORY_SCOPE_AGENT)
                              : ~0ull;
 #pragma unroll
             for (uint32_t k = 0; k < kSbBatch; ++k) {
                 const uint64_t bit = 1ull << (u[k] & 63);
                 const bool fresh = !(old[k] & bit);
                 if (fresh) {
-                    D.dist[u[k]] = depth;
-                    __hip_atomic_fetch_or(&fbn[u[k] >> 6], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    store_through(&D.dist[u[k]], depth);
+                    __hip_atomic_fetch_or(&fbn[u[k] >> 6], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 // one queue reservation per wave
                 const uint64_t m = __ballot(fresh);
@@ -528,84 +528,120 @@ __global__ void k_dl_init(DlDev D, uint32_t src) {
 
 __global__ void k_dl_seed(DlDev D, uint32_t src) { D.dist[src] = 0; }
 
+// Persistent loop, workgroups other than 0 (thread 0): the next command for a
+// workgroup whose last grid level was level1 - 1 (level1 = 0: none yet). The
+// wait gives up after kBarSpinLimit polls without any change of the command
+// word (workgroup 0 stores a heartbeat per level it runs alone), or when
+// another workgroup has raised the abort flag.
+__device__ uint64_t dl_wait_cmd(const DlDev& D, uint32_t level1) {
+    uint64_t seen = ~0ull, spins = 0;
+    for (uint32_t poll = 0;; ++poll) {
+        const uint64_t c = load_coherent(D.cmd);
+        const uint32_t kind = dl_cmd_kind(c);
+        if (kind == kCmdDone || kind == kCmdAbort) return c;
+        if (kind == kCmdGrid && dl_cmd_level1(c) > level1) return c;
+        if (c != seen) {
+            seen = c;
+            spins = 0;
+        } else if (++spins > kBarSpinLimit || ((poll & 15) == 0 && load_coherent(D.abort))) {
+            __hip_atomic_store(D.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return dl_cmd(kCmdAbort, 0, 0);
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Persistent loop, workgroup 0 (every thread): wait until `target` group
+// completions have reached the top counter. False when the wait gave up (then
+// the abort flag is raised) or another workgroup raised it.
+__device__ bool dl_wait_arrivals(const DlDev& D, uint32_t target) {
+    __shared__ uint32_t s_ok;
+    if (threadIdx.x == 0) {
+        uint32_t ok = 1;
+        uint64_t spins = 0;
+        while (load_coherent(&D.bar[kBarTop]) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > kBarSpinLimit || ((spins & 15) == 0 && load_coherent(D.abort))) {
+                __hip_atomic_store(D.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+        }
+        asm volatile("buffer_inv sc0" ::: "memory");  // this CU's L1
+        s_ok = ok;
+    }
+    __syncthreads();
+    const bool ok = s_ok != 0;
+    __syncthreads();
+    return ok;
+}
+
+// Workgroup 0 of the persistent loop: the only decider (dl_protocol.hpp).
+__device__ __noinline__ void dl_decider(const DlDev& D, DlBlock& B, SbShared& S, float* mlp_lds) {
+    const double tick_ms = D.ds->tick_ms;
+    const uint32_t groups = dl_groups(gridDim.x, kBarGroups);
+    if (threadIdx.x == 0) {
+        B.st = D.ds->st;
+        B.pv = D.ds->pv;
+        B.t_prev = 0;
+        B.prev_action = 0;
+    }
+    __syncthreads();
+    if (B.pv.mlp)
+        for (uint32_t i = threadIdx.x; i < GGPU_MLP_PARAMS; i += blockDim.x) mlp_lds[i] = B.pv.mlp[i];
+    __syncthreads();
+    bool more = dl_decide(D, B, false, nullptr, 0, true, tick_ms, mlp_lds);
+    for (uint32_t g = 0; more; ++g) {
+        if (sb_ok(B.st)) {
+            more = sb_levels(D, B, tick_ms, mlp_lds, S);
+            // every store and atomic of those levels has completed before the
+            // command below lets the grid read the frontier and visited words
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (!more) break;
+        }
+        const uint32_t level = B.st.level, action = B.st.action;
+        if (threadIdx.x == 0) store_through(D.cmd, dl_cmd(kCmdGrid, level, action));
+        unsigned long long* slot = D.part + (size_t)dl_part_slot(g) * gridDim.x * kDlParts;
+        dl_level(D, level, action, slot);
+        grid_arrive(D.bar);
+        if (!dl_wait_arrivals(D, (g + 1) * groups)) {
+            if (threadIdx.x == 0) store_through(D.cmd, dl_cmd(kCmdAbort, level, action));
+            return;
+        }
+        more = dl_decide(D, B, true, slot, gridDim.x, true, tick_ms, mlp_lds);
+    }
+    if (threadIdx.x == 0) {
+        store_through(D.cmd, dl_cmd(kCmdDone, B.st.level, 0));
+        D.ds->st = B.st;
+        D.ds->levels = B.st.level;
+        D.ds->done = 1;
+    }
+}
+
 __global__ void __launch_bounds__(kDlThreads) k_dl_persistent(DlDev D) {
     __shared__ float s_mlp[GGPU_MLP_PARAMS];
     __shared__ DlBlock s_b;
     __shared__ SbShared s_sb;
-    __shared__ uint32_t s_more;
-    const double tick_ms = D.ds->tick_ms;
-    if (threadIdx.x == 0) {
-        s_b.st = D.ds->st;
-        s_b.pv = D.ds->pv;
-        s_b.t_prev = 0;
-        s_b.prev_action = 0;
+    __shared__ uint64_t s_cmd;
+    if (blockIdx.x == 0) {
+        dl_decider(D, s_b, s_sb, s_mlp);
+        return;
     }
-    __syncthreads();
-    if (s_b.pv.mlp)
-        for (uint32_t i = threadIdx.x; i < GGPU_MLP_PARAMS; i += blockDim.x) s_mlp[i] = s_b.pv.mlp[i];
-    __syncthreads();
-    const bool writer = blockIdx.x == 0;
-    bool started = false;
-    uint32_t episodes = 0;
-    for (;;) {
-        // the counters of level L (still s_b.st.level before the fold) are in slot (L + 1) & 1
-        if (!dl_decide(D, s_b, started, D.part + (size_t)((s_b.st.level + 1) & 1) * gridDim.x * kDlParts,
-                       gridDim.x, writer, tick_ms, s_mlp))
-            break;
-        started = true;
-        if (sb_ok(s_b.st)) {  // the same decision in every workgroup
-            ++episodes;
-            if (blockIdx.x == 0) {
-                // acquire: visited / frontier words other XCDs changed at the coherence point
-                if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                __syncthreads();
-                const bool more = sb_levels(D, s_b, tick_ms, s_mlp, s_sb);
-                __builtin_amdgcn_s_waitcnt(0);
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    D.ds->st = s_b.st;
-                    D.ds->prev_action = s_b.prev_action;
-                    D.ds->sb_more = more ? 1u : 0u;
-                    // release: this XCD's L2 (visited, frontier, dist, state) to the coherence point
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                    __hip_atomic_fetch_add(D.sb_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                if (!more) break;
-            } else {
-                if (threadIdx.x == 0) {
-                    uint64_t spins = 0;
-                    s_more = 1;
-                    while (load_coherent(D.sb_flag) < episodes) {
-                        __builtin_amdgcn_s_sleep(8);
-                        if (++spins > kBarSpinLimit || load_coherent(D.abort)) {
-                            __hip_atomic_store(D.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            s_more = 2;
-                            break;
-                        }
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    if (s_more == 1) {
-                        s_more = D.ds->sb_more;
-                        s_b.st = D.ds->st;
-                        s_b.prev_action = D.ds->prev_action;
-                    }
-                }
-                __syncthreads();
-                const uint32_t more = s_more;
-                __syncthreads();  // s_more is rewritten at the next episode
-                if (more == 2) return;
-                if (!more) break;
-            }
+    // the others run exactly the grid levels workgroup 0 publishes, in order
+    uint32_t level1 = 0;
+    for (uint32_t g = 0;; ++g) {
+        if (threadIdx.x == 0) {
+            s_cmd = dl_wait_cmd(D, level1);
+            asm volatile("buffer_inv sc0" ::: "memory");  // this CU's L1
         }
-        // level L on the whole grid (decided above, or by workgroup 0 in its last level)
-        const uint32_t level = s_b.st.level, action = s_b.st.action;
-        dl_level(D, level, action, D.part + (size_t)((level + 1) & 1) * gridDim.x * kDlParts);
-        if (!grid_barrier<kAcquireL1>(D.bar, D.abort)) return;
-    }
-    if (writer && threadIdx.x == 0) {
-        D.ds->st = s_b.st;
-        D.ds->levels = s_b.st.level;
-        D.ds->done = 1;
+        __syncthreads();
+        const uint64_t c = s_cmd;
+        __syncthreads();  // s_cmd is rewritten by the next wait
+        if (dl_cmd_kind(c) != kCmdGrid) return;
+        level1 = dl_cmd_level1(c);
+        dl_level(D, level1 - 1, dl_cmd_action(c), D.part + (size_t)dl_part_slot(g) * gridDim.x * kDlParts);
+        grid_arrive(D.bar);
     }
 }
 
@@ -702,7 +738,7 @@ void BfsEngine::device_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, s
     R.fb.alloc(3ull * nb + 1);
     R.part.alloc(2ull * std::max(R.persist_grid, R.level_grid) * kDlParts);
     R.ds.alloc(1);
-    R.ctl.alloc(kBarWords + 8);
+    R.ctl.alloc(kBarWords + 16);
     const size_t tcap = trace ? (size_t)n + 2 : 0;
     if (trace) R.trace.alloc(tcap);
     const float* mlp = (o.policy == GGPU_POLICY_LEARNED && ctx->policy_set[GGPU_POLICY_SINGLE])
@@ -719,7 +755,7 @@ void BfsEngine::device_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, s
     GGPU_HIP_TRY(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device));
     h.tick_ms = clock_khz > 0 ? 1.0 / clock_khz : 0.0;
     GGPU_HIP_TRY(hipMemcpyAsync(R.ds.p, &h, sizeof h, hipMemcpyHostToDevice, s));
-    GGPU_HIP_TRY(hipMemsetAsync(R.ctl.p, 0, (kBarWords + 8) * sizeof(uint32_t), s));
+    GGPU_HIP_TRY(hipMemsetAsync(R.ctl.p, 0, (kBarWords + 16) * sizeof(uint32_t), s));
 
     DlDev D{};
     D.rp = rp.p;
@@ -733,7 +769,7 @@ void BfsEngine::device_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, s
     D.trace = trace ? R.trace.p : nullptr;
     D.bar = R.ctl.p;
     D.abort = R.ctl.p + kBarWords;
-    D.sb_flag = R.ctl.p + kBarWords + 1;
+    D.cmd = (unsigned long long*)(R.ctl.p + kBarWords + 2);  // 8-byte aligned: kBarWords is even
     D.n_heavy = n_heavy;
     D.n = n;
     D.nb = nb;
@@ -746,12 +782,12 @@ void BfsEngine::device_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, s
     GGPU_HIP_TRY(hipGetLastError());
     DlState out{};
     if (o.loop == GGPU_LOOP_PERSISTENT) {
-        // A plain launch: the barrier is the kernel's own (grid_sync.hpp), so it
-        // needs no grid-sync resource from the runtime (hipLaunchCooperativeKernel
+        // A plain launch: the kernel synchronises itself (dl_protocol.hpp), so
+        // it needs no grid-sync resource from the runtime (hipLaunchCooperativeKernel
         // has failed here with "cannot be performed in the present state" while
         // another process held the device). Residency: one workgroup per CU (its
         // LDS footprint allows no second), grid = CUs; a workgroup that cannot
-        // start ends the launch through the bounded barrier (GGPU_ETIMEOUT).
+        // start ends the launch through the bounded waits (GGPU_ETIMEOUT).
         void* args[] = {&D};
         GGPU_HIP_TRY(hipLaunchKernel((const void*)k_dl_persistent, dim3(R.persist_grid), dim3(kDlThreads), args, 0, s));
         GGPU_HIP_TRY(hipEventRecord(R.ev[1], s));
@@ -760,15 +796,17 @@ void BfsEngine::device_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, s
         GGPU_HIP_TRY(hipMemcpyAsync(&out, R.ds.p, sizeof out, hipMemcpyDeviceToHost, s));
         GGPU_HIP_TRY(hipStreamSynchronize(s));
         if (abort) {
-            // where the grid stood: arrivals per barrier group, top counter,
-            // generation, single-workgroup episodes handed back
-            std::vector<uint32_t> c(kBarWords + 8);
+            // where the grid stood: arrivals per group, group completions, the
+            // last command word (kind, level)
+            std::vector<uint32_t> c(kBarWords + 16);
             GGPU_HIP_TRY(hipMemcpy(c.data(), R.ctl.p, c.size() * 4, hipMemcpyDeviceToHost));
-            std::string msg = "persistent BFS: grid barrier timed out (grid " + std::to_string(R.persist_grid) +
+            uint64_t cmd = 0;
+            std::memcpy(&cmd, &c[kBarWords + 2], sizeof cmd);
+            std::string msg = "persistent BFS: a wait timed out (grid " + std::to_string(R.persist_grid) +
                               ", group arrivals";
             for (uint32_t g = 0; g < kBarGroups; ++g) msg += " " + std::to_string(c[g * kBarStride]);
-            msg += ", top " + std::to_string(c[kBarTop]) + ", generation " + std::to_string(c[kBarGen]) +
-                   ", episodes " + std::to_string(c[kBarWords + 1]) + ", level " + std::to_string(out.st.level) + ")";
+            msg += ", group completions " + std::to_string(c[kBarTop]) + ", command kind " +
+                   std::to_string(dl_cmd_kind(cmd)) + " level " + std::to_string((int64_t)dl_cmd_level1(cmd) - 1) + ")";
             throw ArgError(msg, GGPU_ETIMEOUT);
         }
     } else {

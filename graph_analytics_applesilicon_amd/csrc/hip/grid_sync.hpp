@@ -24,6 +24,9 @@
 // The wait is bounded: after `spin_limit` polls the block raises `abort` and
 // leaves; every block returns false from its next barrier, so a launch whose
 // grid is not co-resident (or any bug that loses an arrival) still ends.
+// The fences above serve the batched BFS (kAcquireL2: plain loads after the
+// barrier); the persistent single-source loop uses only grid_arrive and
+// coherent accesses (dl_protocol.hpp).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -92,6 +95,20 @@ __device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t* abort, uin
     }
     __syncthreads();
     return load_coherent(abort) == 0u;
+}
+
+// The arrival half of grid_barrier, for loops in which one workgroup alone
+// observes completion (it polls bar[kBarTop] for rounds * groups arrivals)
+// and releases the others by a command of its own. Called by every thread.
+__device__ __forceinline__ void grid_arrive(uint32_t* bar) {
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's stores and atomics have completed
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t G = gridDim.x;
+        const uint32_t g = blockIdx.x % kBarGroups;
+        const uint32_t gsize = G / kBarGroups + (g < G % kBarGroups ? 1u : 0u);
+        if ((atomicAdd(&bar[g * kBarStride], 1u) + 1) % gsize == 0) atomicAdd(&bar[kBarTop], 1u);
+    }
 }
 
 }  // namespace ggpu

@@ -25,8 +25,10 @@
  *                      (networkx.pagerank with personalization={s:1}). Pinned by
  *                      networkx, by the isolated-seed closed form r = e_s, and by
  *                      the 2-vertex closed form worked out in the tests.
- *   or_check_ranks     src/cpu/pagerank_cpu.hpp:37-60 (pure arithmetic). Pinned by
- *                      hand-computed cases in the tests.
+ *   or_check_ranks     src/cpu/pagerank_cpu.hpp:37-60 (pure arithmetic). Pinned
+ *                      against the compiled reference checkRanks (bit-equal L1,
+ *                      sum and top-1000 error on tie-free inputs; within the
+ *                      extremes the tie choice allows otherwise).
  *   or_mlp_*           FP64 forward/backward of the 5->32->32->3 ReLU MLP with
  *                      mean-squared-error loss (no reference counterpart). Pinned
  *                      by central finite differences in the tests.
@@ -224,14 +226,17 @@ void or_ppr(uint64_t n, const uint32_t* in_rp, const uint32_t* in_ci, const uint
 
 static const double* g_sort_ref;
 static int by_ref_desc(const void* a, const void* b) {
-    double x = g_sort_ref[*(const uint32_t*)a], y = g_sort_ref[*(const uint32_t*)b];
+    const uint32_t i = *(const uint32_t*)a, j = *(const uint32_t*)b;
+    double x = g_sort_ref[i], y = g_sort_ref[j];
     if (x > y) return -1;
     if (x < y) return 1;
-    return 0;
+    return i < j ? -1 : (i > j ? 1 : 0);
 }
 
 /* L1 distance, max relative error over the reference's 1000 largest entries,
- * and the sum of `got` (FP32 values widened to double). */
+ * and the sum of `got` (FP32 values widened to double). Equal reference
+ * values that straddle rank 1000 are taken in id order (std::partial_sort in
+ * the reference leaves that choice unspecified). */
 void or_check_ranks(uint64_t n, const double* ref, const float* got, double* l1, double* max_rel_top, double* sum) {
     double a = 0, s = 0, mx = 0;
     uint64_t k = n < 1000 ? n : 1000;

@@ -92,6 +92,8 @@ def ref_lib() -> Optional[C.CDLL]:
         L.ref_check_ranks.restype = None
         L.ref_gs_no_worse.argtypes = [C.c_double] * 6
         L.ref_gs_no_worse.restype = C.c_int
+        L.ref_ranks_ok.argtypes = [C.c_double] * 3
+        L.ref_ranks_ok.restype = C.c_int
         L.ref_free.argtypes = [C.c_void_p]
         L.ref_free.restype = None
         _ref = L
@@ -322,3 +324,13 @@ def ref_check_ranks(ref, got):
     ref_lib().ref_check_ranks(len(ref), _p(ref, C.c_double), _p(got, C.c_float), C.byref(l1), C.byref(mx),
                               C.byref(s))
     return l1.value, mx.value, s.value
+
+
+def ref_ranks_ok(l1, max_rel, s) -> bool:
+    """The reference's PrCheck::ok (compiled)."""
+    return bool(ref_lib().ref_ranks_ok(l1, max_rel, s))
+
+
+def ref_gs_no_worse(gs, jac) -> bool:
+    """The reference's gsNoWorseThanJacobi (compiled); gs/jac = (l1, max_rel, sum)."""
+    return bool(ref_lib().ref_gs_no_worse(gs[0], gs[1], gs[2], jac[0], jac[1], jac[2]))

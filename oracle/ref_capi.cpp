@@ -180,6 +180,14 @@ void ref_check_ranks(uint64_t n, const double* ref, const float* got, double* l1
     *sum = c.sum;
 }
 
+int ref_ranks_ok(double l1, double rel, double sum) {
+    cpu::PrCheck c;
+    c.l1 = l1;
+    c.maxRelTop = rel;
+    c.sum = sum;
+    return c.ok() ? 1 : 0;
+}
+
 int ref_gs_no_worse(double gl1, double grel, double gsum, double jl1, double jrel, double jsum) {
     cpu::PrCheck g, j;
     g.l1 = gl1;

@@ -44,6 +44,37 @@ def out_degrees(rp):
     return np.diff(rp).astype(np.uint32)
 
 
+def acceptance_inputs():
+    """Seeded (ref, got) pairs for checkRanks; tests/test_oracle_cpu.py rebuilds the same ones.
+    Sizes around the top-1000 cut, and one vector with 60 equal reference values
+    straddling rank 1000 whose errors differ."""
+    out = []
+    for n in (1, 7, 999, 1000, 1001, 4096):
+        rng = np.random.default_rng(1000 + n)
+        ref = rng.random(n) + 0.01
+        ref /= ref.sum()
+        got = (ref * (1 + rng.normal(0, 1e-4, n))).astype(np.float32)
+        out.append((f"random{n}", ref, got))
+    rng = np.random.default_rng(77)
+    ref = np.concatenate([1.0 + rng.random(970), np.full(60, 0.5), 0.4 * rng.random(1970)])
+    ref /= ref.sum()
+    err = rng.normal(0, 1e-5, len(ref))
+    err[970:1030] = np.linspace(1e-3, 2e-3, 60)[rng.permutation(60)]  # the tied values' errors differ
+    got = (ref * (1 + err)).astype(np.float32)
+    out.append(("ties_at_1000", ref, got))
+    return out
+
+
+def acceptance_points():
+    """(l1, rel, sum) triples on and beside each threshold of PrCheck::ok and
+    gsNoWorseThanJacobi, and Jacobi references for the latter."""
+    vals = [0.0, 5e-5, np.nextafter(1e-4, 0), 1e-4, np.nextafter(1e-4, 1), 1.5e-4, 2e-4]
+    rels = [0.0, np.nextafter(1e-3, 0), 1e-3, np.nextafter(1e-3, 1), 1.5e-3 * (1 + 1e-4), 2e-3]
+    pts = [(l1, rel, 1.0 + ds) for l1 in vals for rel in rels for ds in (0.0, -9e-5, 1e-4, -1e-4, 1.5e-4, 2e-4)]
+    jacs = [(5e-5, 5e-4, 1.0), (1.5e-4, 1.5e-3, 1.0 - 1e-5), (1e-4, 1e-3, 1.0)]
+    return pts, jacs
+
+
 def main():
     if O.ref_lib() is None:
         sys.exit("oracle/_ref/libggpu_ref.so is not built (make -C oracle)")
@@ -86,6 +117,13 @@ def main():
             gap, same = O.ref_locality(qrp, qci, qfl)
             fx["orderings"].append({"graph": name, "ordering": ordering, "seed": seed, "sha1": sha(p, "<u4"),
                                     "head": p[:8].tolist(), "mean_gap": gap, "same_block64": same})
+    fx["check_ranks"] = []
+    for name, ref, got in acceptance_inputs():
+        l1, mx, sm = O.ref_check_ranks(ref, got)
+        fx["check_ranks"].append({"name": name, "l1": float.hex(l1), "max_rel": float.hex(mx), "sum": float.hex(sm)})
+    pts, jacs = acceptance_points()
+    fx["ranks_ok"] = [int(O.ref_ranks_ok(*p)) for p in pts]
+    fx["gs_no_worse"] = [[int(O.ref_gs_no_worse(p, j)) for p in pts] for j in jacs]
     path = os.path.join(HERE, "ref_vectors.json")
     with open(path, "w") as f:
         json.dump(fx, f, indent=0, sort_keys=True)

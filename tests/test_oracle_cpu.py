@@ -211,6 +211,63 @@ def test_check_ranks_definition():
     assert O.ranks_ok(9e-5, 9e-4, 1 + 9e-5) and not O.ranks_ok(1e-4, 0, 1) and not O.ranks_ok(0, 1e-3, 1)
 
 
+def _golden_inputs():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(os.path.dirname(__file__), "golden",
+                                                                               "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    return mg
+
+
+def _tie_extremes(ref, got, k=1000):
+    """Smallest and largest top-k relative error over every choice of the equal
+    reference values that straddle rank k."""
+    order = np.argsort(-ref, kind="stable")
+    cut = ref[order[k - 1]]
+    above = order[ref[order] > cut]
+    tied = np.flatnonzero(ref == cut)
+    need = k - len(above)
+    err = np.abs(ref - got.astype(np.float64)) / ref
+    base = err[above].max() if len(above) else 0.0
+    e = np.sort(err[tied])
+    return max(base, e[need - 1]), max(base, e[-1])
+
+
+def test_check_ranks_equals_the_reference():
+    """or_check_ranks against the reference's compiled checkRanks (pagerank_cpu.hpp:37-60),
+    through golden vectors it wrote: L1 and sum bit for bit, the top-1000 error bit for bit
+    where no tie straddles rank 1000, and within the extremes of the tie choice where one does
+    (std::partial_sort leaves it unspecified; the oracle takes ids in order)."""
+    mg = _golden_inputs()
+    want = {x["name"]: x for x in GOLDEN["check_ranks"]}
+    for name, ref, got in mg.acceptance_inputs():
+        l1, mx, s = O.check_ranks(ref, got)
+        w = want[name]
+        assert float.hex(l1) == w["l1"] and float.hex(s) == w["sum"], name
+        if name.startswith("ties"):
+            lo, hi = _tie_extremes(ref, got)
+            assert lo < hi and lo <= mx <= hi and lo <= float.fromhex(w["max_rel"]) <= hi, (name, lo, mx, hi)
+        else:
+            assert float.hex(mx) == w["max_rel"], name
+        if O.ref_lib() is not None:
+            assert O.ref_check_ranks(ref, got)[0] == l1
+
+
+def test_acceptance_rules_equal_the_reference():
+    """ranks_ok and gs_no_worse against the reference's PrCheck::ok and gsNoWorseThanJacobi
+    (pagerank_cpu.hpp:31, :80-84) on and beside every threshold (golden booleans)."""
+    mg = _golden_inputs()
+    pts, jacs = mg.acceptance_points()
+    assert [int(O.ranks_ok(*p)) for p in pts] == GOLDEN["ranks_ok"]
+    for j, want in zip(jacs, GOLDEN["gs_no_worse"]):
+        assert [int(O.gs_no_worse(p, j)) for p in pts] == want, j
+    assert 0 < sum(GOLDEN["ranks_ok"]) < len(pts) and all(0 < sum(w) < len(pts) for w in GOLDEN["gs_no_worse"])
+    if O.ref_lib() is not None:
+        assert [int(O.ref_ranks_ok(*p)) for p in pts] == GOLDEN["ranks_ok"]
+        assert [int(O.ref_gs_no_worse(p, jacs[1])) for p in pts] == GOLDEN["gs_no_worse"][1]
+
+
 # ------------------------------------------------------------------- MLP
 def _tiny(seed):
     rng = np.random.default_rng(seed)

@@ -2,12 +2,13 @@
 (src/bench/main.cpp): exit codes, the CSV schema, --validate for every
 variant, and --ordering (relabel, run, map back, validate).
 
-The CPU cases run everywhere; the GPU cases (marked) run every GPU variant
-of the binary with --validate, which compares against the reference's own
-rules (exact dist for BFS, checkRanks for PageRank). The device level loops
-are checked under the original numbering only: an ordering is a host-side
-relabel that the other variants already cover. Each case starts fresh
-processes, so this module is collected after the in-process GPU tests.
+The CPU cases run everywhere; the GPU cases (marked, one gbench process per
+case) run every GPU variant of the binary with --validate, which compares
+against the reference's own rules (exact dist for BFS, checkRanks for
+PageRank). The device level loops are checked under the original numbering
+only: an ordering is a host-side relabel that the other variants already
+cover. Each case starts a fresh process, so this module is collected after
+the in-process GPU tests.
 """
 import csv
 import os
@@ -59,32 +60,33 @@ def test_cpu_variants_validate_under_every_ordering(tmp_path, ordering):
     assert {r["ordering"] for r in lrows} == {ordering} and all(float(r["mean_gap"]) >= 0 for r in lrows)
 
 
+BFS_GRAPHS = ("synth:rmat12", "synth:grid100x100", "synth:star100k")
+PR_GRAPHS = ("synth:rmat12_directed", "synth:star100k")
+# one gbench run per case: (ordering, graph, algo, what, extra arguments)
+GPU_CASES = (
+    [(o, g, "bfs", v, ["--variant", v, "--sources", "3"]) for o in ORDERINGS for g in BFS_GRAPHS
+     for v in ("gpu_vertex", "gpu_naive", "gpu_binned", "gpu_edge", "gpu_do", "hybrid")]
+    # the device level loops under the original numbering (an ordering is a host-side relabel)
+    + [("original", g, "bfs", "gpu_do-" + loop, ["--variant", "gpu_do", "--loop", loop, "--sources", "3"])
+       for g in BFS_GRAPHS for loop in ("graph", "persistent")]
+    + [(o, g, "msbfs", "batch-" + b, ["--sources", "70", "--batch", "64", "--batch-order", b]) for o in ORDERINGS
+       for g in BFS_GRAPHS for b in ("auto", "degree", "given")]
+    + [(o, g, "pr", v, ["--variant", v]) for o in ORDERINGS for g in PR_GRAPHS
+       for v in ("gpu_push", "gpu_pull_scalar", "gpu_pull_vector", "gpu_pull_adaptive")]
+    + [(o, g, "pr", "gpu_pull_gs", ["--variant", "gpu_pull_gs", "--tol", "1e-6", "--ref-tol", "1e-10"])
+       for o in ORDERINGS for g in PR_GRAPHS])
+
+
 @pytest.mark.gpu
 @needs_binary
-@pytest.mark.parametrize("ordering", ORDERINGS)
-def test_gpu_variants_validate_under_every_ordering(tmp_path, ordering):
+@pytest.mark.parametrize("ordering,graph,algo,what,extra", GPU_CASES,
+                         ids=["-".join((c[0], c[1][6:], c[2], c[3])) for c in GPU_CASES])
+def test_gpu_variant_validates(tmp_path, ordering, graph, algo, what, extra):
     out = tmp_path / "r.csv"
-    common = ["--warmup", "1", "--trials", "2", "--validate", "--csv", str(out), "--ordering", ordering]
-    bfs = ["gpu_vertex", "gpu_naive", "gpu_binned", "gpu_edge", "gpu_do", "hybrid"]
-    for graph in ("synth:rmat12", "synth:grid100x100", "synth:star100k"):
-        for variant in bfs:
-            r = gbench("--graph", graph, "--algo", "bfs", "--variant", variant, "--sources", "3", *common)
-            assert r.returncode == 0 and "validated OK" in r.stdout, (graph, variant, r.stdout + r.stderr)
-        for loop in ("graph", "persistent") if ordering == "original" else ():
-            r = gbench("--graph", graph, "--algo", "bfs", "--variant", "gpu_do", "--loop", loop, "--sources", "3",
-                       *common)
-            assert r.returncode == 0 and "validated OK" in r.stdout, (graph, loop, r.stdout + r.stderr)
-        for border in ("auto", "degree", "given"):
-            r = gbench("--graph", graph, "--algo", "msbfs", "--sources", "70", "--batch", "64", "--batch-order",
-                       border, *common)
-            assert r.returncode == 0 and "validated OK" in r.stdout, (graph, "msbfs", border, r.stdout + r.stderr)
-    for graph in ("synth:rmat12_directed", "synth:star100k"):
-        for variant in ("gpu_push", "gpu_pull_scalar", "gpu_pull_vector", "gpu_pull_adaptive"):
-            r = gbench("--graph", graph, "--algo", "pr", "--variant", variant, *common)
-            assert r.returncode == 0 and "validated OK" in r.stdout, (graph, variant, r.stdout + r.stderr)
-        r = gbench("--graph", graph, "--algo", "pr", "--variant", "gpu_pull_gs", "--tol", "1e-6", "--ref-tol",
-                   "1e-10", *common)
-        assert r.returncode == 0 and "validated OK" in r.stdout, (graph, "gs", r.stdout + r.stderr)
+    print(f"gbench {graph} {algo} {what} --ordering {ordering}", flush=True)
+    r = gbench("--graph", graph, "--algo", algo, *extra, "--warmup", "1", "--trials", "2", "--validate", "--csv",
+               str(out), "--ordering", ordering)
+    assert r.returncode == 0 and "validated OK" in r.stdout, r.stdout + r.stderr
     rows = list(csv.DictReader(open(out)))
-    assert {r["ordering"] for r in rows} == {ordering}
-    assert all(r["validated"] == "1" for r in rows)
+    assert rows and {x["ordering"] for x in rows} == {ordering}
+    assert all(x["validated"] == "1" for x in rows)
