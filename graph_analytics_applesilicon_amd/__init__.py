@@ -41,7 +41,7 @@ POLICY_HEURISTIC, POLICY_LEARNED, POLICY_SCRIPTED = 0, 1, 2
 BATCH_ORDER_GIVEN, BATCH_ORDER_DEGREE, BATCH_ORDER_AUTO = 0, 1, 2
 ACTION_TD_EDGE, ACTION_TD_BINNED, ACTION_BU = 0, 1, 2
 PR_SCALAR, PR_VECTOR, PR_ADAPTIVE, PR_GS, PR_PUSH = 1, 2, 3, 4, 5
-PR_LAYOUT_STREAM, PR_LAYOUT_BINS, PR_LAYOUT_HOT = 0, 1, 2
+PR_LAYOUT_STREAM, PR_LAYOUT_BINS, PR_LAYOUT_HOT, PR_LAYOUT_TILED = 0, 1, 2, 3
 PR_ORDER_GIVEN, PR_ORDER_DEGREE = 0, 1
 PR_VARIANTS = {"gpu_pull_scalar": PR_SCALAR, "gpu_pull_vector": PR_VECTOR,
                "gpu_pull_adaptive": PR_ADAPTIVE, "gpu_pull_gs": PR_GS, "gpu_push": PR_PUSH}

@@ -14,6 +14,7 @@
 #pragma once
 
 #include "dev.hpp"
+#include "level_policy.hpp"
 
 namespace ggpu {
 
@@ -36,8 +37,6 @@ constexpr int kStatSlots = 4;     // ring of per-level stats slots
 constexpr int kOwnerStage = 2048; // LDS-staged owner range of the edge expand
 constexpr int kScanItems = 16;    // elements per thread of the tile scan
 
-// Frontier representation present on the device.
-enum Fmt : uint32_t { FMT_QUEUE = 0, FMT_BINS = 1, FMT_BITMAP = 2 };
 
 struct BfsDev {
     const uint32_t* rp;

@@ -24,8 +24,8 @@
 // In the persistent loop every datum another workgroup reads during the
 // launch (counters, frontier and visited words, the command word) is written
 // with device-scope atomics or write-through stores and read with coherent
-// loads; the read-only CSR stays cached. dist is only written with
-// write-through stores (the host reads it after the launch).
+// loads; the read-only CSR stays cached. dist has no reader during the
+// launch (the host reads it afterwards); each entry is written once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -346,18 +346,21 @@ __device__ bool dl_decide(const DlDev& D, DlBlock& B, bool started, const unsign
 // vertex map in LDS, then every edge's col_idx load and visited atomic is
 // issued at once, so a level is about three memory round trips. The decisions
 // come from the same fold / decide code with the same counters (found,
-// next_edges), so the trace equals the other loops'; the frontier bitmaps are
-// kept exactly as the grid levels keep them (read with coherent loads, bits
-// set by device-scope atomics, slots cleared by write-through stores), so
-// either side can take over at any level.
+// next_edges), so the trace equals the other loops'. The frontier bitmaps:
+// on entry the frontier is read out of its slot into LDS and the three slots
+// are cleared (write-through); the levels then keep their frontiers in LDS
+// and set no bitmap bits, except for vertices beyond the queue's capacity
+// (which ends the run of single-workgroup levels); on leaving, the queued
+// frontier's bits are added to its slot, so the grid finds the slots as it
+// keeps them (frontier in slot L % 3, slot (L + 1) % 3 clear).
 constexpr uint32_t kSbBatch = 8;  // edges per lane in flight (kSbEdges / blockDim rounds at most 2)
 
 struct SbShared {
-    uint32_t q[3][kSbMax];    // frontier queues: current, next, previous (whose bits are cleared)
+    uint32_t q[2][kSbMax];    // frontier queues: current, next
     uint32_t off[kSbMax];     // exclusive prefix of the current frontier's degrees
     uint32_t vb[kSbMax];      // row start of each current frontier vertex
     uint16_t map[kSbEdges];   // frontier index of each edge of the level
-    uint32_t qn[3];           // queue counts (may exceed kSbMax: then the queue is cut)
+    uint32_t qn[2];           // queue counts (may exceed kSbMax: then the queue is cut)
     uint64_t wsum[2][kDlThreads / kWave];
 };
 
@@ -389,20 +392,21 @@ __device__ __forceinline__ uint64_t sb_scan(uint64_t v, uint64_t& excl, uint64_t
 }
 
 // Workgroup 0 only. On entry B.st is decided for level L = B.st.level with
-// sb_ok(B.st); fb slot L % 3 holds the frontier, slot (L + 1) % 3 is clear,
-// slot (L + 2) % 3 holds the previous frontier. Runs levels until the
-// traversal ends (returns false) or a level is decided that sb_ok rejects
-// (returns true with B.st decided for that level and the slots as on entry).
+// sb_ok(B.st); fb slot L % 3 holds the frontier, slot (L + 1) % 3 is clear.
+// Runs levels until the traversal ends (returns false) or a level is decided
+// that sb_ok rejects (returns true with B.st decided for that level L', its
+// frontier in slot L' % 3 and the other two slots clear).
 __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const float* mlp_lds, SbShared& S) {
     const uint32_t tid = threadIdx.x, lane = lane_id();
-    if (tid < 3) S.qn[tid] = 0;
+    if (tid < 2) S.qn[tid] = 0;
     __syncthreads();
-    {  // the frontier from its bitmap; the previous frontier's slot cleared in full
-        const uint64_t* fb = D.fb + (size_t)(B.st.level % 3) * D.nb;
-        uint64_t* fold = D.fb + (size_t)((B.st.level + 2) % 3) * D.nb;
+    {  // the frontier out of its slot; all three slots cleared
+        uint64_t* fb = D.fb + (size_t)(B.st.level % 3) * D.nb;
+        uint64_t* f2 = D.fb + (size_t)((B.st.level + 2) % 3) * D.nb;
         for (uint32_t i = tid; i < D.nb; i += blockDim.x) {
             uint64_t w = load_coherent(&fb[i]);
-            store_through(&fold[i], 0ull);
+            if (w) store_through(&fb[i], 0ull);
+            store_through(&f2[i], 0ull);
             while (w) {
                 const uint32_t b = (uint32_t)__ffsll((long long)w) - 1;
                 w &= w - 1;
@@ -415,8 +419,8 @@ __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const floa
     __syncthreads();
     uint64_t cut_edges = 0;  // degrees of vertices found beyond the queue's capacity (this thread's)
     for (uint32_t cur = 0, first = 1;; first = 0) {
-        const uint32_t nxt = (cur + 1) % 3, prv = (cur + 2) % 3;
-        const uint32_t found = S.qn[cur], qn = min(found, kSbMax), pn = min(S.qn[prv], kSbMax);
+        const uint32_t nxt = cur ^ 1u;
+        const uint32_t found = S.qn[cur], qn = min(found, kSbMax);
         // the frontier's degrees: four consecutive vertices per thread
         uint32_t b4[4], d4[4];
         uint64_t dsum = 0;
@@ -452,17 +456,22 @@ __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const floa
         __syncthreads();
         if (!first) {
             if (!dl_decide_rest(D, B, true, mlp_lds)) return false;
-            if (found > kSbMax || !sb_ok(B.st)) return true;
+            if (found > kSbMax || !sb_ok(B.st)) {
+                // hand back: the queued part of the frontier into its slot (the
+                // vertices beyond the queue's capacity set their bits when found)
+                uint64_t* fbl = D.fb + (size_t)(B.st.level % 3) * D.nb;
+                for (uint32_t i = tid; i < qn; i += blockDim.x) {
+                    const uint32_t v = S.q[cur][i];
+                    __hip_atomic_fetch_or(&fbl[v >> 6], 1ull << (v & 63), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                return true;
+            }
         }
         cut_edges = 0;
         const uint32_t level = B.st.level, depth = level + 1;
-        if (tid == 0) store_through(D.cmd, dl_cmd(kCmdNone, level, B.st.action));  // heartbeat
-        uint64_t* fbn = D.fb + (size_t)((level + 1) % 3) * D.nb;
-        uint64_t* fold = D.fb + (size_t)((level + 2) % 3) * D.nb;
-        for (uint32_t i = tid; i < pn; i += blockDim.x) store_through(&fold[S.q[prv][i] >> 6], 0ull);
-        // every thread's clears (and its previous level's stores and atomics) land
-        // before any of this level's atomics on the slot just cleared
-        __builtin_amdgcn_s_waitcnt(0);
+        // heartbeat for the waiting workgroups' bounded wait
+        if (tid == 0 && (level & 63) == 0) store_through(D.cmd, dl_cmd(kCmdNone, level, B.st.action));
+        uint64_t* fbn = D.fb + (size_t)((level + 1) % 3) * D.nb;  // bits of vertices beyond the queue only
         if (tid == 0) S.qn[nxt] = 0;
         __syncthreads();
         // edge-parallel expansion: edge e0 + k * blockDim (consecutive edges in a wave), kSbBatch in flight
@@ -490,8 +499,9 @@ __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const floa
                 const uint64_t bit = 1ull << (u[k] & 63);
                 const bool fresh = !(old[k] & bit);
                 if (fresh) {
-                    store_through(&D.dist[u[k]], depth);
-                    __hip_atomic_fetch_or(&fbn[u[k] >> 6], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    // no workgroup reads dist during the launch: a plain store,
+                    // written back by the end of the kernel
+                    D.dist[u[k]] = depth;
                 }
                 // one queue reservation per wave
                 const uint64_t m = __ballot(fresh);
@@ -501,13 +511,16 @@ __device__ bool sb_levels(const DlDev& D, DlBlock& B, double tick_ms, const floa
                 base = __shfl(base, __ffsll((long long)m) - 1, kWave);
                 if (fresh) {
                     const uint32_t p = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
-                    if (p < kSbMax) S.q[nxt][p] = u[k];
-                    else cut_edges += D.rp[u[k] + 1] - D.rp[u[k]];
+                    if (p < kSbMax) {
+                        S.q[nxt][p] = u[k];
+                    } else {  // beyond the queue: the bitmap holds it (this ends the single-workgroup levels)
+                        cut_edges += D.rp[u[k] + 1] - D.rp[u[k]];
+                        __hip_atomic_fetch_or(&fbn[u[k] >> 6], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
                 }
             }
         }
-        // this level's stores and atomics are waited for after the next level's clears
-        __syncthreads();
+        __syncthreads();  // the next queue is complete
         cur = nxt;
     }
 }
@@ -531,7 +544,7 @@ __global__ void k_dl_seed(DlDev D, uint32_t src) { D.dist[src] = 0; }
 // Persistent loop, workgroups other than 0 (thread 0): the next command for a
 // workgroup whose last grid level was level1 - 1 (level1 = 0: none yet). The
 // wait gives up after kBarSpinLimit polls without any change of the command
-// word (workgroup 0 stores a heartbeat per level it runs alone), or when
+// word (workgroup 0 stores a heartbeat every 64 levels it runs alone), or when
 // another workgroup has raised the abort flag.
 __device__ uint64_t dl_wait_cmd(const DlDev& D, uint32_t level1) {
     uint64_t seen = ~0ull, spins = 0;
@@ -547,7 +560,10 @@ __device__ uint64_t dl_wait_cmd(const DlDev& D, uint32_t level1) {
             __hip_atomic_store(D.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return dl_cmd(kCmdAbort, 0, 0);
         }
-        __builtin_amdgcn_s_sleep(2);
+        // while workgroup 0 runs levels alone, poll less often: 255 pollers of one
+        // line load the memory channel workgroup 0's own accesses go through
+        if (kind == kCmdNone) __builtin_amdgcn_s_sleep(32);
+        else __builtin_amdgcn_s_sleep(4);
     }
 }
 

@@ -17,7 +17,7 @@
 //     every workgroup's counters with coherent loads, folds and decides.
 // Levels that workgroup 0 runs alone (small top-down frontiers) are not grid
 // levels: the others keep polling; workgroup 0 stores a heartbeat word (kind
-// NONE, the current level) at each such level so their bounded wait counts
+// NONE, the current level) every 64 such levels so their bounded wait counts
 // only polls without progress. The traversal ends with DONE; a workgroup that
 // gives up waiting raises the abort flag and every waiter leaves on it.
 // Part slot g & 1 is written only after command g, which workgroup 0 stores
@@ -26,7 +26,8 @@
 // level g + 1 off the lines being read at level g).
 // Every datum another workgroup reads during the launch is written with a
 // device-scope atomic or a write-through store and read with a coherent load;
-// nothing depends on cache writeback or invalidate fences.
+// nothing depends on cache writeback or invalidate fences. (The depths have
+// no reader during the launch.)
 #pragma once
 
 #include "bfs_engine.hpp"

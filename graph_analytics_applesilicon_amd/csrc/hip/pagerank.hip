@@ -26,6 +26,8 @@
 #include "graph.hpp"
 #include "pagerank_kernels.hpp"
 #include "pr_shared.hpp"
+#include "pr_tiled.hpp"
+#include "pr_tiling.hpp"
 
 namespace ggpu {
 
@@ -39,6 +41,8 @@ __global__ void k_pr_sweep(PrSweep P) { pr_sweep_body(P); }
 __global__ void __launch_bounds__(kHotThreads) k_pr_sweep_hot(PrSweep P) { pr_hot_sweep_body(P); }
 
 __global__ void k_pr_longfix(PrSweep P) { pr_longfix_body(P); }
+__global__ void __launch_bounds__(kTileThreads) k_pr_sweep_tiled(PrSweep P, PrTiledDev Z) { pr_tiled_body(P, Z); }
+__global__ void __launch_bounds__(256) k_pr_tiled_merge(PrSweep P, PrTiledDev Z) { pr_tiled_merge_body(P, Z); }
 __global__ void k_pr_reduce(const double* partial, uint32_t cap, uint32_t count, PrAcc* out) {
     pr_reduce_body(partial, cap, count, out);
 }
@@ -136,6 +140,44 @@ struct StreamLayout {
     bool multi = false;  // some long row has more than one chunk
 };
 
+// The TILED layout's plan (pr_tiling.hpp) on the device.
+struct TiledLayout {
+    uint32_t min_indeg = 0, R = 0, K = 0, groups = 0;
+    uint64_t dense_arcs = 0, cold_arcs = 0, untiled_arcs = 0;
+    DBuf<uint4> items, segs;
+    DBuf<TileTask> tasks;
+    DBuf<uint2> wg;
+    DBuf<uint16_t> tci, ent_len;
+    DBuf<uint32_t> cci, ent_row, trow, optr, oidx;
+    DBuf<float2> part, ovf;
+    PrTiledDev dev() const {
+        PrTiledDev Z{};
+        Z.items = items.p;
+        Z.tasks = tasks.p;
+        Z.wg_tasks = wg.p;
+        Z.segs = segs.p;
+        Z.tci = tci.p;
+        Z.cci = cci.p;
+        Z.ent_row = ent_row.p;
+        Z.ent_len = ent_len.p;
+        Z.part = part.p;
+        Z.ovf = ovf.p;
+        Z.optr = optr.p;
+        Z.oidx = oidx.p;
+        Z.trow = trow.p;
+        Z.R = R;
+        Z.K = K;
+        return Z;
+    }
+};
+
+template <class T, class H>
+static void upload_vec(DBuf<T>& d, const std::vector<H>& h) {
+    static_assert(sizeof(T) == sizeof(H), "host and device records differ in size");
+    d.alloc(std::max<size_t>(h.size(), 1));
+    if (!h.empty()) GGPU_HIP_TRY(hipMemcpy(d.p, h.data(), h.size() * sizeof(H), hipMemcpyHostToDevice));
+}
+
 // One vertex numbering of the in-CSR with its device copy and work layouts.
 // Spaces: the caller's ids; descending out-degree (hot contributions packed
 // into few cache lines); the Gauss-Seidel colour order (colour sets are
@@ -152,6 +194,7 @@ struct Space {
     DBuf<uint32_t> rp, ci, odd;
     std::vector<std::unique_ptr<StreamLayout>> stream;  // one per range
     std::unique_ptr<StreamLayout> wstream;              // HOT layout: wave-sized items over all rows
+    std::unique_ptr<TiledLayout> tiled;                 // TILED layout
     DBuf<uint32_t> bins;
     uint32_t bin_count[3] = {0, 0, 0};
     // gpu_push: out-arcs (the transpose of `in`), sources of out-degree >= kPushHeavy
@@ -235,6 +278,7 @@ struct PrEngine {
         for (uint32_t v = 0; v < n; ++v) S.id_of[v] = perm[base.id_of.empty() ? v : base.id_of[v]];
         S.stream.clear();
         S.wstream.reset();
+        S.tiled.reset();
         S.bins_ready = false;
         S.out_ready = false;
     }
@@ -351,6 +395,45 @@ struct PrEngine {
 
     void build_wave_stream(Space& S) {
         if (!S.wstream) S.wstream = make_stream(S, 0, n, kWaveNnz, kWaveNnz, kWaveNnz);
+    }
+
+    // The TILED plan of space S for rows of in-degree >= min_indeg.
+    void build_tiled(Space& S, uint32_t min_indeg) {
+        if (S.tiled && S.tiled->min_indeg == min_indeg) return;
+        TiledParams prm;
+        prm.min_indeg = min_indeg;
+        prm.tile = kTileSrc;
+        prm.nnz_cap = kTileNnz;
+        prm.rows_cap = kTileRows;
+        prm.slots = kTileSlots;
+        prm.groups = (uint32_t)std::max(ctx->cus, 1);
+        const TiledPlan plan = plan_tiled(S.in, prm);
+        auto T = std::make_unique<TiledLayout>();
+        T->min_indeg = min_indeg;
+        T->R = plan.R;
+        T->K = plan.K;
+        T->groups = prm.groups;
+        T->dense_arcs = plan.dense_arcs;
+        T->cold_arcs = plan.cold_arcs;
+        T->untiled_arcs = plan.untiled_arcs;
+        upload_vec(T->items, plan.items);
+        upload_vec(T->tasks, plan.tasks);
+        upload_vec(T->wg, plan.wg);
+        upload_vec(T->segs, plan.segs);
+        upload_vec(T->tci, plan.tci);
+        upload_vec(T->cci, plan.cci);
+        upload_vec(T->ent_row, plan.ent_row);
+        upload_vec(T->ent_len, plan.ent_len);
+        upload_vec(T->trow, plan.trow);
+        upload_vec(T->optr, plan.optr);
+        upload_vec(T->oidx, plan.oidx);
+        // partials of rows without arcs from a tile stay zero: cleared once here
+        const size_t np = (size_t)(plan.K + 1) * plan.R;
+        T->part.alloc(std::max<size_t>(np, 1));
+        T->ovf.alloc(std::max<size_t>(plan.ovf_row.size(), 1));
+        GGPU_HIP_TRY(hipMemset(T->part.p, 0, std::max<size_t>(np, 1) * sizeof(float2)));
+        GGPU_HIP_TRY(hipDeviceSynchronize());
+        S.tiled = std::move(T);
     }
 
     void build_bins(Space& S, uint32_t small_max, uint32_t large_min) {
@@ -528,6 +611,24 @@ struct PrEngine {
         P.r_old = want_delta ? r.p + (size_t)c * n : nullptr;
         P.r_new = r.p + (size_t)nx * n;
         P.acc_in = acc.p + c;
+        if (variant == GGPU_PR_ADAPTIVE && o.layout == GGPU_PR_LAYOUT_TILED) {
+            // the tile sweep (partials and the untiled rows), then the merge of the tiled rows
+            build_tiled(S, o.large_min);
+            const TiledLayout& T = *S.tiled;
+            const PrTiledDev Z = T.dev();
+            const uint32_t gm = blocks(T.R, 256);
+            if (npart + T.groups + gm > kPartialCap) throw ArgError("too many blocks in one sweep");
+            P.partial_base = npart;
+            npart += T.groups;
+            k_pr_sweep_tiled<<<T.groups, kTileThreads, 0, ctx->stream>>>(P, Z);
+            GGPU_HIP_TRY(hipGetLastError());
+            P.partial_base = npart;
+            npart += gm;
+            k_pr_tiled_merge<<<gm, 256, 0, ctx->stream>>>(P, Z);
+            GGPU_HIP_TRY(hipGetLastError());
+            reduce(acc.p + nx);
+            return;
+        }
         const uint32_t grid = jacobi_layout(P, S, variant, o);
         launch(P, grid, o.tg);
         reduce(acc.p + nx);
@@ -581,7 +682,9 @@ struct PrEngine {
         const uint32_t L = o.medium_lanes;
         if (L == 0 || L > 64 || (L & (L - 1))) throw ArgError("medium_lanes must be a power of two in [1, 64]");
         if (o.iters < 0) throw ArgError("iters must be >= 0");
-        if (o.layout > GGPU_PR_LAYOUT_HOT) throw ArgError("unknown PageRank layout");
+        if (o.layout > GGPU_PR_LAYOUT_TILED) throw ArgError("unknown PageRank layout");
+        if (o.layout == GGPU_PR_LAYOUT_TILED && (o.large_min == 0 || o.large_min > kTileNnz))
+            throw ArgError("TILED needs 1 <= large_min <= " + std::to_string(kTileNnz) + " (the in-degree of a tiled row)");
         if (o.order > GGPU_PR_ORDER_DEGREE) throw ArgError("unknown PageRank vertex order");
         const int base_id = o.order == GGPU_PR_ORDER_DEGREE ? SP_DEGREE : SP_GIVEN;
         int sid = base_id;
@@ -684,7 +787,10 @@ struct PrEngine {
         std::vector<Cand> cands = {{GGPU_PR_LAYOUT_STREAM, GGPU_PR_ORDER_GIVEN, 0, 0, 64},
                                    {GGPU_PR_LAYOUT_STREAM, GGPU_PR_ORDER_DEGREE, 0, 0, 64},
                                    {GGPU_PR_LAYOUT_HOT, GGPU_PR_ORDER_GIVEN, 0, 0, 64},
-                                   {GGPU_PR_LAYOUT_HOT, GGPU_PR_ORDER_DEGREE, 0, 0, 64}};
+                                   {GGPU_PR_LAYOUT_HOT, GGPU_PR_ORDER_DEGREE, 0, 0, 64},
+                                   {GGPU_PR_LAYOUT_TILED, GGPU_PR_ORDER_DEGREE, 0, 64, 64},
+                                   {GGPU_PR_LAYOUT_TILED, GGPU_PR_ORDER_DEGREE, 0, 128, 64},
+                                   {GGPU_PR_LAYOUT_TILED, GGPU_PR_ORDER_DEGREE, 0, 256, 64}};
         for (uint32_t order : {(uint32_t)GGPU_PR_ORDER_GIVEN, (uint32_t)GGPU_PR_ORDER_DEGREE})
             for (uint32_t sm : {4u, 16u})
                 for (uint32_t lg : {256u, 4096u})
@@ -699,6 +805,8 @@ struct PrEngine {
                 t.small_max = c.sm;
                 t.large_min = c.lg;
                 t.medium_lanes = c.L;
+            } else if (c.layout == GGPU_PR_LAYOUT_TILED) {
+                t.large_min = c.lg;
             }
             t.iters = iters;
             t.tol = 0;

@@ -1,150 +1,318 @@
-// CPU BFS variants and the hybrid BFS's CPU level steps.
-//   bfs_serial  src/cpu/bfs_cpu.cpp:27-45   (FIFO BFS, the bit-exact oracle of gbench --validate)
-//   bfs_omp     src/cpu/bfs_cpu.cpp:207-223 (level-synchronous top-down, runtime schedule)
-//   bfs_do      src/cpu/bfs_cpu.cpp:227-283 (direction-optimizing, GAP alpha/beta rule)
-//   steps::*    src/cpu/bfs_steps.cpp:27-143, with u64 bitmap words (the GPU layout here)
+// CPU BFS variants and the hybrid BFS's CPU levels.
+//   bfs_serial  src/cpu/bfs_cpu.cpp:27-45   FIFO BFS (gbench --validate's truth)
+//   bfs_omp     src/cpu/bfs_cpu.cpp:207-223 every level top-down (cpu_omp)
+//   bfs_do      src/cpu/bfs_cpu.cpp:227-283 direction-optimizing (cpu_do)
+//   steps::*    src/cpu/bfs_steps.cpp:27-143, the same contract on the GPU's
+//               buffer layout (queue + degrees top-down, u64 bitmaps bottom-up)
+//
+// Design:
+//  * a frontier is a bitmap of u64 words with one summary bit per word: a
+//    frontier is walked, cleared and collected through its summary, touching
+//    only its non-zero words;
+//  * cpu_omp / cpu_do discover a vertex with the fetch_or on the visited
+//    bitmap that sets its bit (exactly once; dist gets a plain store), and a
+//    level sets the next frontier's bits; no per-thread queues;
+//  * every queue is collected from a bitmap in increasing vertex order, so a
+//    level's output does not depend on thread timing;
+//  * work is dealt in units (words of 64 vertices, or queue entries) by one
+//    atomic cursor: a static split, chunks of 64 units, or guided halving
+//    (--schedule static|dynamic|guided);
+//  * cpu_do's direction per level comes from the GPU engine's own state
+//    machine (fold_level / decide_action, level_policy.hpp: the GAP rule),
+//    so its level sequence is the one every GPU loop follows.
 #include "graph.hpp"
 
 #include <omp.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+
+#include "level_policy.hpp"
 
 namespace ggpu {
 
 namespace {
 
-int nthreads(int t) { return t > 0 ? t : omp_get_max_threads(); }
+int team(int t) { return t > 0 ? t : omp_get_max_threads(); }
 
 double now_ms() { return omp_get_wtime() * 1e3; }
 
-// First writer wins: a relaxed load filters visited vertices before the CAS.
-inline bool claim(uint32_t* dist, uint32_t v, uint32_t d) {
-    uint32_t seen = __atomic_load_n(&dist[v], __ATOMIC_RELAXED);
-    if (seen != kInf) return false;
-    return __atomic_compare_exchange_n(&dist[v], &seen, d, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED);
-}
-
-struct Level {
-    uint64_t next = 0, scout = 0, examined = 0;
-};
-
-// Discoveries are buffered per thread and published with one atomic per block.
-constexpr size_t kFlush = 4096;
-
-Level td_step(const Csr& g, uint32_t* dist, const uint32_t* front, int64_t fsize, uint32_t* next, uint32_t d, int nt) {
-    uint64_t tail = 0, scout = 0, examined = 0;
-#pragma omp parallel num_threads(nt) reduction(+ : scout, examined)
-    {
-        std::vector<uint32_t> buf;
-        buf.reserve(kFlush);
-        auto publish = [&] {
-            if (buf.empty()) return;
-            const uint64_t at = __atomic_fetch_add(&tail, (uint64_t)buf.size(), __ATOMIC_RELAXED);
-            std::memcpy(next + at, buf.data(), buf.size() * 4);
-            buf.clear();
-        };
-#pragma omp for schedule(runtime) nowait
-        for (int64_t i = 0; i < fsize; ++i) {
-            const uint32_t u = front[i];
-            examined += g.deg(u);
-            for (const uint32_t* p = g.nbr_begin(u); p != g.nbr_end(u); ++p)
-                if (claim(dist, *p, d)) {
-                    scout += g.deg(*p);
-                    buf.push_back(*p);
-                    if (buf.size() == kFlush) publish();
-                }
-        }
-        publish();
-    }
-    return {tail, scout, examined};
-}
-
-struct Bits {
-    std::vector<uint64_t> w;
-    explicit Bits(uint64_t n) : w((n + 63) / 64, 0) {}
-    bool test(uint32_t i) const { return (w[i >> 6] >> (i & 63)) & 1; }
-    void set_atomic(uint32_t i) { __atomic_fetch_or(&w[i >> 6], 1ull << (i & 63), __ATOMIC_RELAXED); }
-    void zero(int nt) {
-        const int64_t nw = (int64_t)w.size();
-#pragma omp parallel for num_threads(nt) schedule(static)
-        for (int64_t i = 0; i < nw; ++i) w[i] = 0;
+struct Counts {
+    uint64_t found = 0, examined = 0, next_edges = 0;
+    void add(const Counts& o) {
+        found += o.found;
+        examined += o.examined;
+        next_edges += o.next_edges;
     }
 };
 
-// Bottom-up: each unvisited vertex stops at its first neighbour in the frontier.
-Level bu_step(const Csr& g, uint32_t* dist, const Bits& front, Bits& next, uint32_t d, int nt) {
-    uint64_t awake = 0, examined = 0;
-    const int64_t n = (int64_t)g.n;
-#pragma omp parallel for num_threads(nt) schedule(dynamic, 1024) reduction(+ : awake, examined)
-    for (int64_t v = 0; v < n; ++v) {
-        if (dist[v] != kInf) continue;
-        const uint32_t* b = g.nbr_begin(v);
-        const uint32_t* e = g.nbr_end(v);
-        const uint32_t* p = b;
-        for (; p != e; ++p)
-            if (front.test(*p)) {
-                dist[v] = d;
-                next.set_atomic((uint32_t)v);
-                ++awake;
-                ++p;
-                break;
-            }
-        examined += (uint64_t)(p - b);
+// Runs body(first, end, counts) over [0, units) on nt threads under policy
+// `pol` (Dynamic64: chunks of `chunk` units); returns the summed counts.
+template <class Body>
+Counts deal(int nt, Schedule pol, uint64_t units, Body body, uint64_t chunk_units = 64) {
+    Counts total;
+    if (units == 0) return total;
+    if (nt <= 1 || units < 2) {
+        body(0, units, total);
+        return total;
     }
-    return {awake, 0, examined};
-}
-
-void q2b(const uint32_t* q, int64_t size, Bits& bm, int nt) {
-    bm.zero(nt);
-#pragma omp parallel for num_threads(nt) schedule(static)
-    for (int64_t i = 0; i < size; ++i) bm.set_atomic(q[i]);
-}
-
-int64_t b2q(const Bits& bm, uint64_t n, uint32_t* q, int nt) {
-    uint64_t tail = 0;
-    const int64_t nw = (int64_t)bm.w.size();
+    std::atomic<uint64_t> cursor{0};
 #pragma omp parallel num_threads(nt)
     {
-        std::vector<uint32_t> buf;
-        buf.reserve(kFlush);
-        auto publish = [&] {
-            if (buf.empty()) return;
-            const uint64_t at = __atomic_fetch_add(&tail, (uint64_t)buf.size(), __ATOMIC_RELAXED);
-            std::memcpy(q + at, buf.data(), buf.size() * 4);
-            buf.clear();
-        };
-#pragma omp for schedule(static) nowait
-        for (int64_t i = 0; i < nw; ++i)
-            for (uint64_t word = bm.w[i]; word; word &= word - 1) {
-                const uint64_t v = (uint64_t)i * 64 + (uint64_t)__builtin_ctzll(word);
-                if (v < n) {
-                    buf.push_back((uint32_t)v);
-                    if (buf.size() == kFlush) publish();
+        Counts mine;
+        const uint64_t t = (uint64_t)omp_get_thread_num(), T = (uint64_t)omp_get_num_threads();
+        if (pol == Schedule::Static) {
+            body(units * t / T, units * (t + 1) / T, mine);
+        } else {
+            for (;;) {
+                uint64_t chunk = chunk_units;
+                if (pol == Schedule::Guided) {
+                    const uint64_t seen = cursor.load(std::memory_order_relaxed);
+                    chunk = std::max<uint64_t>(chunk_units, (units - std::min(units, seen)) / (2 * T));
+                }
+                const uint64_t b = cursor.fetch_add(chunk, std::memory_order_relaxed);
+                if (b >= units) break;
+                body(b, std::min(units, b + chunk), mine);
+            }
+        }
+#pragma omp critical(ggpu_cpu_bfs_counts)
+        total.add(mine);
+    }
+    return total;
+}
+
+inline bool test_bit(const uint64_t* w, uint32_t v) { return (w[v >> 6] >> (v & 63)) & 1ull; }
+
+// A vertex set as words plus a summary bit per word (bit i%64 of sum[i/64]:
+// word i is not zero), so walking, clearing and collecting a sparse set touch
+// only its words.
+struct Frontier {
+    uint64_t n = 0, nw = 0;
+    std::vector<uint64_t> w, sum;
+    explicit Frontier(uint64_t n_ = 0) { resize(n_); }
+    void resize(uint64_t n_) {
+        n = n_;
+        nw = (n + 63) / 64;
+        w.assign(nw, 0);
+        sum.assign((nw + 63) / 64, 0);
+    }
+    // inserts; Shared: other threads insert concurrently
+    template <bool Shared>
+    void insert(uint32_t v) {
+        uint64_t old;
+        if (Shared) {
+            old = __atomic_fetch_or(&w[v >> 6], 1ull << (v & 63), __ATOMIC_RELAXED);
+        } else {
+            old = w[v >> 6];
+            w[v >> 6] = old | 1ull << (v & 63);
+        }
+        if (old == 0) {
+            if (Shared) __atomic_fetch_or(&sum[v >> 12], 1ull << ((v >> 6) & 63), __ATOMIC_RELAXED);
+            else sum[v >> 12] |= 1ull << ((v >> 6) & 63);
+        }
+    }
+    // a whole word written by its only writer
+    void put_word(uint64_t i, uint64_t bits) {
+        w[i] = bits;
+        if (bits) __atomic_fetch_or(&sum[i >> 6], 1ull << (i & 63), __ATOMIC_RELAXED);
+    }
+    uint64_t words_set() const {
+        uint64_t c = 0;
+        for (uint64_t x : sum) c += (uint64_t)__builtin_popcountll(x);
+        return c;
+    }
+    // sparse sets are cleared and collected by one thread
+    int team_for(int nt) const { return words_set() <= 4096 ? 1 : nt; }
+    void clear(int nt) {
+        deal(team_for(nt), Schedule::Dynamic64, sum.size(), [&](uint64_t b, uint64_t e, Counts&) {
+            for (uint64_t s = b; s < e; ++s) {
+                for (uint64_t bits = sum[s]; bits; bits &= bits - 1) w[s * 64 + (uint64_t)__builtin_ctzll(bits)] = 0;
+                sum[s] = 0;
+            }
+        }, 8);
+    }
+    // the members in increasing order (q) with their degrees (deg, optional)
+    uint64_t collect(const Csr& g, uint32_t* q, uint32_t* deg, int nt) const {
+        nt = team_for(nt);
+        const uint64_t ns = sum.size();
+        std::vector<uint64_t> at(ns + 1, 0);
+        deal(nt, Schedule::Dynamic64, ns, [&](uint64_t b, uint64_t e, Counts&) {
+            for (uint64_t s = b; s < e; ++s) {
+                uint64_t c = 0;
+                for (uint64_t bits = sum[s]; bits; bits &= bits - 1)
+                    c += (uint64_t)__builtin_popcountll(w[s * 64 + (uint64_t)__builtin_ctzll(bits)]);
+                at[s + 1] = c;
+            }
+        }, 8);
+        for (uint64_t s = 0; s < ns; ++s) at[s + 1] += at[s];
+        deal(nt, Schedule::Dynamic64, ns, [&](uint64_t b, uint64_t e, Counts&) {
+            for (uint64_t s = b; s < e; ++s) {
+                uint64_t k = at[s];
+                for (uint64_t bits = sum[s]; bits; bits &= bits - 1) {
+                    const uint64_t i = s * 64 + (uint64_t)__builtin_ctzll(bits);
+                    for (uint64_t m = w[i]; m; m &= m - 1) {
+                        const uint32_t v = (uint32_t)(i * 64 + (uint64_t)__builtin_ctzll(m));
+                        q[k] = v;
+                        if (deg) deg[k] = g.deg(v);
+                        ++k;
+                    }
                 }
             }
-        publish();
+        }, 8);
+        return at[ns];
     }
-    return (int64_t)tail;
+};
+
+// Top-down from the set `cur` into `next`, dealt by summary words (4096
+// vertices each, 8 per chunk); `vis` decides discovery. examined = the
+// frontier's degrees. `cur` is left empty: each word is cleared by the thread
+// that walked it.
+template <bool Shared>
+Counts td_walk(const Csr& g, Frontier& cur, Frontier& next, uint64_t* vis, uint32_t* dist, uint32_t depth, int nt,
+               Schedule pol) {
+    return deal(
+        nt, pol, cur.sum.size(),
+        [&](uint64_t b, uint64_t e, Counts& c) {
+            for (uint64_t s = b; s < e; ++s) {
+                const uint64_t words = cur.sum[s];
+                cur.sum[s] = 0;
+                for (uint64_t wb = words; wb; wb &= wb - 1) {
+                    const uint64_t i = s * 64 + (uint64_t)__builtin_ctzll(wb);
+                    const uint64_t word = cur.w[i];
+                    cur.w[i] = 0;
+                    for (uint64_t bits = word; bits; bits &= bits - 1) {
+                        const uint32_t v = (uint32_t)(i * 64 + (uint64_t)__builtin_ctzll(bits));
+                        c.examined += g.deg(v);
+                        for (const uint32_t* p = g.nbr_begin(v); p != g.nbr_end(v); ++p) {
+                            const uint32_t u = *p;
+                            const uint64_t bit = 1ull << (u & 63);
+                            if (Shared) {
+                                if (__atomic_load_n(&vis[u >> 6], __ATOMIC_RELAXED) & bit) continue;
+                                if (__atomic_fetch_or(&vis[u >> 6], bit, __ATOMIC_RELAXED) & bit) continue;
+                            } else {
+                                if (vis[u >> 6] & bit) continue;
+                                vis[u >> 6] |= bit;
+                            }
+                            dist[u] = depth;
+                            next.insert<Shared>(u);
+                            c.found += 1;
+                            c.next_edges += g.deg(u);
+                        }
+                    }
+                }
+            }
+        },
+        8);
 }
 
-void set_schedule(Schedule s) {
-    if (s == Schedule::Static) omp_set_schedule(omp_sched_static, 0);
-    else if (s == Schedule::Dynamic64) omp_set_schedule(omp_sched_dynamic, 64);
-    else omp_set_schedule(omp_sched_guided, 0);
+Counts td_from_set(const Csr& g, Frontier& cur, Frontier& next, uint64_t* vis, uint32_t* dist, uint32_t depth,
+                   int nt, Schedule pol) {
+    return nt > 1 ? td_walk<true>(g, cur, next, vis, dist, depth, nt, pol)
+                  : td_walk<false>(g, cur, next, vis, dist, depth, 1, pol);
 }
 
-ggpu_level make_level(uint32_t level, uint64_t frontier, uint64_t edges, char dir, double ms) {
+// Bottom-up: each unvisited vertex takes the first neighbour in `cur`; the
+// thread handling word i of the vertex range owns vis[i] and next's word i.
+Counts bu_to_set(const Csr& g, const Frontier& cur, Frontier& next, uint64_t* vis, uint32_t* dist, uint32_t depth,
+                 int nt, Schedule pol) {
+    const uint64_t n = g.n, nw = (n + 63) / 64;
+    return deal(nt, pol, nw, [&](uint64_t b, uint64_t e, Counts& c) {
+        for (uint64_t i = b; i < e; ++i) {
+            uint64_t open = ~vis[i];
+            if (i == nw - 1 && (n & 63)) open &= (1ull << (n & 63)) - 1;
+            uint64_t got = 0;
+            for (; open; open &= open - 1) {
+                const uint32_t v = (uint32_t)(i * 64 + (uint64_t)__builtin_ctzll(open));
+                const uint32_t* p = g.nbr_begin(v);
+                const uint32_t* end = g.nbr_end(v);
+                for (; p != end && !test_bit(cur.w.data(), *p); ++p) {
+                }
+                c.examined += (uint64_t)(p - g.nbr_begin(v)) + (p != end ? 1 : 0);
+                if (p != end) {
+                    got |= 1ull << (v & 63);
+                    dist[v] = depth;
+                    c.next_edges += g.deg(v);
+                }
+            }
+            if (got) {
+                vis[i] |= got;
+                next.put_word(i, got);
+                c.found += (uint64_t)__builtin_popcountll(got);
+            }
+        }
+    });
+}
+
+ggpu_level make_level(uint32_t level, uint64_t frontier, uint64_t edges, bool bottom_up, double ms) {
     ggpu_level L{};
     L.level = level;
-    L.action = dir == 'B' ? 2u : 0u;
+    L.action = bottom_up ? GGPU_ACTION_BU : GGPU_ACTION_TD_EDGE;
     L.frontier = frontier;
     L.edges = edges;
-    L.frontier_edges = dir == 'T' ? edges : 0;
-    L.direction = dir;
+    L.frontier_edges = bottom_up ? 0 : edges;
+    L.direction = bottom_up ? 'B' : 'T';
     L.device = 'C';
     L.time_ms = ms;
     return L;
+}
+
+void fill_inf(uint32_t* dist, uint64_t n, int nt) {
+    deal(nt, Schedule::Static, (n + 4095) / 4096, [&](uint64_t b, uint64_t e, Counts&) {
+        std::fill(dist + b * 4096, dist + std::min<uint64_t>(n, e * 4096), kInf);
+    });
+}
+
+// Level loop of cpu_omp (all top-down) and cpu_do (the GAP state machine).
+void level_loop(const Csr& g, uint32_t src, uint32_t* dist, const CpuBfsOpts& o, std::vector<ggpu_level>* trace,
+                bool direction_optimizing) {
+    const int nt = team(o.threads);
+    const uint64_t n = g.n;
+    fill_inf(dist, n, nt);
+    std::vector<uint64_t> vis((n + 63) / 64, 0);
+    Frontier cur(n), next(n);
+    dist[src] = 0;
+    vis[src >> 6] |= 1ull << (src & 63);
+    cur.insert<false>(src);
+    LoopState st{};
+    st.size = 1;
+    st.total = g.deg(src);
+    st.scout = st.total;
+    st.to_check = g.m;
+    st.unvisited_edges = g.m - st.total;
+    PolicyView pv{};
+    pv.mode = GGPU_POLICY_HEURISTIC;
+    pv.variant = GGPU_BFS_DO;
+    pv.alpha = o.alpha;
+    pv.beta = o.beta;
+    // cpu_do deals chunks of 64 words (the reference's dynamic,64); cpu_omp follows --schedule
+    const Schedule pol = direction_optimizing ? Schedule::Dynamic64 : o.schedule;
+    for (;;) {
+        if (direction_optimizing) decide_action(st, pv, n);
+        else st.action = GGPU_ACTION_TD_EDGE;
+        const bool bu = st.action == GGPU_ACTION_BU;
+        const double t0 = trace ? now_ms() : 0.0;
+        // a top-down level of few edges runs on one thread (no parallel region)
+        const int lnt = !bu && st.total < steps::kSerialEdges ? 1 : nt;
+        const Counts c = bu ? bu_to_set(g, cur, next, vis.data(), dist, st.level + 1, nt, pol)
+                            : td_from_set(g, cur, next, vis.data(), dist, st.level + 1, lnt, pol);
+        if (bu) cur.clear(nt);  // (a top-down level has emptied cur itself)
+        std::swap(cur, next);
+        if (trace) trace->push_back(make_level(st.level, st.size, c.examined, bu, now_ms() - t0));
+        LevelOut r;
+        r.found = c.found;
+        r.examined = c.examined;
+        r.next_edges = c.next_edges;
+        fold_level(st, r, pv, n);
+        if (st.size == 0) break;
+    }
+}
+
+// The calling thread's scratch set of the hybrid's top-down level.
+Frontier& scratch_set(uint64_t n) {
+    static thread_local Frontier f;
+    if (f.n != n) f.resize(n);
+    return f;
 }
 
 }  // namespace
@@ -167,183 +335,117 @@ void bfs_serial(const Csr& g, uint32_t src, uint32_t* dist) {
 }
 
 void bfs_omp(const Csr& g, uint32_t src, uint32_t* dist, const CpuBfsOpts& o, std::vector<ggpu_level>* trace) {
-    const int nt = nthreads(o.threads);
-    set_schedule(o.schedule);
-#pragma omp parallel for num_threads(nt) schedule(static)
-    for (int64_t v = 0; v < (int64_t)g.n; ++v) dist[v] = kInf;
-    std::vector<uint32_t> front(g.n), next(g.n);
-    dist[src] = 0;
-    front[0] = src;
-    int64_t fsize = 1;
-    for (uint32_t level = 0; fsize > 0; ++level) {
-        const double t0 = trace ? now_ms() : 0;
-        const Level r = td_step(g, dist, front.data(), fsize, next.data(), level + 1, nt);
-        if (trace) trace->push_back(make_level(level, (uint64_t)fsize, r.examined, 'T', now_ms() - t0));
-        front.swap(next);
-        fsize = (int64_t)r.next;
-    }
+    level_loop(g, src, dist, o, trace, false);
 }
 
 void bfs_do(const Csr& g, uint32_t src, uint32_t* dist, const CpuBfsOpts& o, std::vector<ggpu_level>* trace) {
-    const int nt = nthreads(o.threads);
-    const uint64_t n = g.n;
-    omp_set_schedule(omp_sched_dynamic, 64);
-#pragma omp parallel for num_threads(nt) schedule(static)
-    for (int64_t v = 0; v < (int64_t)n; ++v) dist[v] = kInf;
-    std::vector<uint32_t> queue(n), next_queue(n);
-    Bits front(n), next(n);
-    dist[src] = 0;
-    queue[0] = src;
-    int64_t fsize = 1;
-    uint32_t level = 0;
-    uint64_t to_check = g.m;      // m_u: edges not yet examined top-down
-    uint64_t scout = g.deg(src);  // m_f: edges of the frontier
-    while (fsize > 0) {
-        if ((double)scout > (double)to_check / o.alpha) {
-            double t0 = now_ms();
-            q2b(queue.data(), fsize, front, nt);
-            double conv = now_ms() - t0;
-            uint64_t awake = (uint64_t)fsize, prev;
-            do {  // bottom-up while the frontier grows or stays above n / beta
-                prev = awake;
-                const double t1 = now_ms();
-                next.zero(nt);
-                const Level r = bu_step(g, dist, front, next, level + 1, nt);
-                if (trace) trace->push_back(make_level(level, prev, r.examined, 'B', now_ms() - t1 + conv));
-                conv = 0;
-                awake = r.next;
-                std::swap(front.w, next.w);
-                ++level;
-            } while (awake >= prev || (double)awake > (double)n / o.beta);
-            t0 = now_ms();
-            fsize = b2q(front, n, queue.data(), nt);
-            if (trace && !trace->empty()) trace->back().time_ms += now_ms() - t0;
-            scout = 1;
-        } else {
-            const double t0 = trace ? now_ms() : 0;
-            to_check -= std::min(to_check, scout);
-            const Level r = td_step(g, dist, queue.data(), fsize, next_queue.data(), level + 1, nt);
-            if (trace) trace->push_back(make_level(level, (uint64_t)fsize, r.examined, 'T', now_ms() - t0));
-            scout = r.scout;
-            queue.swap(next_queue);
-            fsize = (int64_t)r.next;
-            ++level;
-        }
-    }
+    level_loop(g, src, dist, o, trace, true);
 }
 
 namespace steps {
 
 void init_dist(uint32_t* dist, uint64_t n, uint32_t src, int threads) {
-#pragma omp parallel for num_threads(nthreads(threads)) schedule(static) if (n > (1u << 16))
-    for (int64_t v = 0; v < (int64_t)n; ++v) dist[v] = kInf;
+    fill_inf(dist, n, n > (1u << 16) ? team(threads) : 1);
     dist[src] = 0;
 }
 
+// The hybrid mirror keeps no visited bitmap: discovery is the CAS of dist from
+// INF; discoveries go to the thread's scratch set, collected in vertex order.
 Out top_down(const Csr& g, uint32_t* dist, const uint32_t* q, uint64_t size, uint64_t edges, uint32_t* q_next,
              uint32_t* deg_next, uint32_t level, int threads) {
-    Out out;
-    uint64_t tail = 0;
-    const int nt = edges < kSerialEdges ? 1 : nthreads(threads);
-#pragma omp parallel num_threads(nt)
-    {
-        static thread_local std::vector<uint32_t> lq, ld;
-        lq.clear();
-        ld.clear();
-        uint64_t my_edges = 0, my_examined = 0;
-#pragma omp for schedule(dynamic, 64) nowait
-        for (int64_t i = 0; i < (int64_t)size; ++i) {
+    const int nt = edges < kSerialEdges ? 1 : team(threads);
+    Frontier& S = scratch_set(g.n);
+    const uint32_t depth = level + 1;
+    const Counts c = deal(nt, Schedule::Dynamic64, size, [&](uint64_t b, uint64_t e, Counts& k) {
+        for (uint64_t i = b; i < e; ++i) {
             const uint32_t v = q[i];
-            my_examined += g.deg(v);
-            for (const uint32_t* p = g.nbr_begin(v); p != g.nbr_end(v); ++p)
-                if (claim(dist, *p, level + 1)) {
-                    const uint32_t d = g.deg(*p);
-                    lq.push_back(*p);
-                    ld.push_back(d);
-                    my_edges += d;
-                }
+            k.examined += g.deg(v);
+            for (const uint32_t* p = g.nbr_begin(v); p != g.nbr_end(v); ++p) {
+                uint32_t seen = __atomic_load_n(&dist[*p], __ATOMIC_RELAXED);
+                if (seen != kInf ||
+                    !__atomic_compare_exchange_n(&dist[*p], &seen, depth, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED))
+                    continue;
+                S.insert<true>(*p);
+                k.next_edges += g.deg(*p);
+            }
         }
-        const uint64_t at = __atomic_fetch_add(&tail, (uint64_t)lq.size(), __ATOMIC_RELAXED);
-        if (!lq.empty()) {
-            std::memcpy(q_next + at, lq.data(), lq.size() * 4);
-            std::memcpy(deg_next + at, ld.data(), ld.size() * 4);
-        }
-        __atomic_fetch_add(&out.next_edges, my_edges, __ATOMIC_RELAXED);
-        __atomic_fetch_add(&out.examined, my_examined, __ATOMIC_RELAXED);
-    }
-    out.found = tail;
+    });
+    Out out;
+    out.found = S.collect(g, q_next, deg_next, nt);
+    out.examined = c.examined;
+    out.next_edges = c.next_edges;
+    S.clear(nt);
     return out;
 }
 
+// Unvisited = dist INF. Every word of `next` is written (the GPU layout reads it whole).
 Out bottom_up(const Csr& g, uint32_t* dist, const uint64_t* front, uint64_t* next, uint32_t level, int threads) {
-    const uint64_t n = g.n, words = (n + 63) / 64;
-    uint64_t found = 0, examined = 0, next_edges = 0;
-    // One thread owns whole words, so `next` is written without atomics.
-#pragma omp parallel for num_threads(nthreads(threads)) schedule(dynamic, 32) \
-    reduction(+ : found, examined, next_edges)
-    for (int64_t w = 0; w < (int64_t)words; ++w) {
-        uint64_t word = 0;
-        const uint64_t hi = std::min<uint64_t>(n, (uint64_t)(w + 1) * 64);
-        for (uint64_t v = (uint64_t)w * 64; v < hi; ++v) {
-            if (dist[v] != kInf) continue;
-            for (const uint32_t* p = g.nbr_begin(v); p != g.nbr_end(v); ++p) {
-                ++examined;
-                if ((front[*p >> 6] >> (*p & 63)) & 1ull) {
-                    dist[v] = level + 1;
-                    word |= 1ull << (v & 63);
-                    ++found;
-                    next_edges += g.deg(v);
-                    break;
+    const uint64_t n = g.n, nw = (n + 63) / 64;
+    const uint32_t depth = level + 1;
+    const Counts c = deal(team(threads), Schedule::Dynamic64, nw, [&](uint64_t b, uint64_t e, Counts& k) {
+        for (uint64_t i = b; i < e; ++i) {
+            const uint64_t lo = i * 64, hi = std::min<uint64_t>(n, lo + 64);
+            uint64_t open = 0;
+            for (uint64_t v = lo; v < hi; ++v) open |= (uint64_t)(dist[v] == kInf) << (v - lo);
+            uint64_t got = 0;
+            for (; open; open &= open - 1) {
+                const uint32_t v = (uint32_t)(lo + (uint64_t)__builtin_ctzll(open));
+                const uint32_t* p = g.nbr_begin(v);
+                const uint32_t* end = g.nbr_end(v);
+                for (; p != end && !test_bit(front, *p); ++p) {
+                }
+                k.examined += (uint64_t)(p - g.nbr_begin(v)) + (p != end ? 1 : 0);
+                if (p != end) {
+                    got |= 1ull << (v - lo);
+                    dist[v] = depth;
+                    k.next_edges += g.deg(v);
                 }
             }
+            next[i] = got;
+            k.found += (uint64_t)__builtin_popcountll(got);
         }
-        next[w] = word;
-    }
-    return {found, examined, next_edges};
+    });
+    Out out;
+    out.found = c.found;
+    out.examined = c.examined;
+    out.next_edges = c.next_edges;
+    return out;
 }
 
 void queue_to_bitmap(const uint32_t* q, uint64_t size, uint64_t* bm, uint64_t n, int threads) {
-    const uint64_t words = (n + 63) / 64;
-    const int nt = size < kSerialEdges ? 1 : nthreads(threads);
-#pragma omp parallel num_threads(nt)
-    {
-#pragma omp for schedule(static)
-        for (int64_t w = 0; w < (int64_t)words; ++w) bm[w] = 0;
-#pragma omp for schedule(static)
-        for (int64_t i = 0; i < (int64_t)size; ++i)
-            __atomic_fetch_or(&bm[q[i] >> 6], 1ull << (q[i] & 63), __ATOMIC_RELAXED);
-    }
+    const int nt = size < kSerialEdges ? 1 : team(threads);
+    std::memset(bm, 0, ((n + 63) / 64) * 8);
+    deal(nt, Schedule::Dynamic64, size, [&](uint64_t b, uint64_t e, Counts&) {
+        for (uint64_t i = b; i < e; ++i) __atomic_fetch_or(&bm[q[i] >> 6], 1ull << (q[i] & 63), __ATOMIC_RELAXED);
+    });
 }
 
+// Increasing vertex order: popcounts of fixed 4096-word spans, their prefix, then the writes.
 uint64_t bitmap_to_queue(const Csr& g, const uint64_t* bm, uint32_t* q, uint32_t* deg, int threads) {
-    const uint64_t words = (g.n + 63) / 64;
-    const int nt = nthreads(threads);
-    std::vector<uint64_t> start(nt + 1, 0);
-    int team = 1;
-    // Count per static word range, prefix, then write: increasing vertex order.
-#pragma omp parallel num_threads(nt)
-    {
-        const int t = omp_get_thread_num(), T = omp_get_num_threads();
-        const uint64_t lo = words * t / T, hi = words * (t + 1) / T;
-        uint64_t c = 0;
-        for (uint64_t w = lo; w < hi; ++w) c += (uint64_t)__builtin_popcountll(bm[w]);
-        start[t + 1] = c;
-#pragma omp barrier
-#pragma omp single
-        {
-            team = T;
-            for (int i = 0; i < T; ++i) start[i + 1] += start[i];
+    const uint64_t nw = (g.n + 63) / 64, spans = (nw + 4095) / 4096;
+    const int nt = team(threads);
+    std::vector<uint64_t> at(spans + 1, 0);
+    deal(nt, Schedule::Dynamic64, spans, [&](uint64_t b, uint64_t e, Counts&) {
+        for (uint64_t s = b; s < e; ++s) {
+            uint64_t c = 0;
+            for (uint64_t i = s * 4096; i < std::min(nw, s * 4096 + 4096); ++i) c += (uint64_t)__builtin_popcountll(bm[i]);
+            at[s + 1] = c;
         }
-        uint64_t at = start[t];
-        for (uint64_t w = lo; w < hi; ++w)
-            for (uint64_t bits = bm[w]; bits; bits &= bits - 1) {
-                const uint32_t v = (uint32_t)(w * 64 + (uint64_t)__builtin_ctzll(bits));
-                q[at] = v;
-                deg[at] = g.deg(v);
-                ++at;
-            }
-    }
-    return start[team];
+    });
+    for (uint64_t s = 0; s < spans; ++s) at[s + 1] += at[s];
+    deal(nt, Schedule::Dynamic64, spans, [&](uint64_t b, uint64_t e, Counts&) {
+        for (uint64_t s = b; s < e; ++s) {
+            uint64_t k = at[s];
+            for (uint64_t i = s * 4096; i < std::min(nw, s * 4096 + 4096); ++i)
+                for (uint64_t bits = bm[i]; bits; bits &= bits - 1) {
+                    const uint32_t v = (uint32_t)(i * 64 + (uint64_t)__builtin_ctzll(bits));
+                    q[k] = v;
+                    deg[k] = g.deg(v);
+                    ++k;
+                }
+        }
+    });
+    return at[spans];
 }
 
 }  // namespace steps

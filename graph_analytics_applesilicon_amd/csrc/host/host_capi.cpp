@@ -304,3 +304,35 @@ void ggpu_check_ranks(uint64_t n, const double* ref, const float* got, double* l
 }
 
 }  // extern "C"
+
+// ---- test hooks (include/ggpu_testing.h)
+#include "ggpu_testing.h"
+#include "pr_tiling.hpp"
+
+extern "C" int ggpu_test_pr_tiled_emulate(uint64_t n, uint64_t m, const uint32_t* in_row_ptr, const uint32_t* in_col_idx,
+                                          const uint32_t* params, const float* contrib, double* sums, uint64_t* stats) {
+    if (!in_row_ptr || (m && !in_col_idx) || !params || !contrib || !sums || !stats) return fail(GGPU_EINVAL, "null argument");
+    return guarded([&] {
+        const ggpu::Csr in = ggpu::from_arrays(n, m, in_row_ptr, in_col_idx, GGPU_FLAG_SORTED);
+        ggpu::TiledParams p;
+        p.min_indeg = params[0];
+        p.tile = params[1];
+        p.nnz_cap = params[2];
+        p.rows_cap = params[3];
+        p.slots = params[4];
+        p.dense_min_arcs = params[5];
+        p.task_arcs = params[6];
+        p.task_items = params[7];
+        p.groups = params[8];
+        const ggpu::TiledPlan plan = ggpu::plan_tiled(in, p);
+        ggpu::emulate_tiled(in, p, plan, contrib, sums);
+        stats[0] = plan.R;
+        stats[1] = plan.K;
+        stats[2] = plan.dense_arcs;
+        stats[3] = plan.cold_arcs;
+        stats[4] = plan.untiled_arcs;
+        stats[5] = plan.items.size();
+        stats[6] = plan.tasks.size();
+        stats[7] = plan.segs.size();
+    });
+}

@@ -162,6 +162,12 @@ typedef struct ggpu_pr_opts {
  * are the most gathered ones), wave-sized row blocks of <= 512 entries.
  * Jacobi sweeps only: in-place sweeps with this layout use STREAM. */
 #define GGPU_PR_LAYOUT_HOT 2
+/* TILED: rows of in-degree >= large_min (1..1024) are summed per tile of 32768
+ * consecutive sources whose contributions are held in LDS, one compensated
+ * partial per (tile, row), merged per row in tile order by a second kernel;
+ * the other rows as in HOT. Built for ORDER_DEGREE (hub rows and hot sources
+ * first). Jacobi sweeps only: in-place sweeps with this layout use STREAM. */
+#define GGPU_PR_LAYOUT_TILED 3
 void ggpu_pr_opts_default(ggpu_pr_opts* o);
 /* "gpu_pull_scalar" | "gpu_pull_vector" | "gpu_pull_adaptive" | "gpu_pull_gs" | "gpu_push" -> GGPU_PR_* */
 int ggpu_pr_parse_variant(const char* name, int* variant);

@@ -1,0 +1,26 @@
+/* Test hooks of the host library (no reference counterpart; not part of the
+ * drop-in boundary). Callers: tests/ only. */
+#ifndef GGPU_TESTING_H
+#define GGPU_TESTING_H
+
+#include "ggpu_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Builds the TILED PageRank work plan for an in-CSR and executes it on the
+ * CPU in FP64: sums[v] = sum of contrib[u] over v's in-arcs, taken item by
+ * item as the GPU takes them. params (9 words): min in-degree of a tiled row,
+ * sources per tile, arcs per item, segments per run, chunk slots per task,
+ * min arcs of a dense tile, arcs per task, items per task, workgroups.
+ * stats (8 words): tiled rows, dense tiles, dense arcs, cold arcs, untiled
+ * arcs, items, tasks, long segments. Checks the plan's invariants: a
+ * violation returns GGPU_EINVAL with the message in ggpu_host_last_error(). */
+int ggpu_test_pr_tiled_emulate(uint64_t n, uint64_t m, const uint32_t* in_row_ptr, const uint32_t* in_col_idx,
+                               const uint32_t* params, const float* contrib, double* sums, uint64_t* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -283,7 +283,7 @@ static Co<int> barrier(Sim& S, uint32_t b) {
 static Co<int> episode(Sim& S, LoopState& st, bool heartbeat, std::vector<std::pair<uint32_t, uint32_t>>& trace) {
     for (;;) {
         co_await S.y(0);
-        if (heartbeat) S.store_through(0, A_CMD, dl_cmd(kCmdNone, st.level, st.action));
+        if (heartbeat && (st.level & 63) == 0) S.store_through(0, A_CMD, dl_cmd(kCmdNone, st.level, st.action));
         trace.emplace_back(st.level, st.action | 0x100u);
         LevelOut r;
         r.found = st.level + 1 < S.size.size() ? S.size[st.level + 1] : 0;

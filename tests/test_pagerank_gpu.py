@@ -32,6 +32,10 @@ CFGS = [("gpu_pull_scalar", {}), ("gpu_pull_vector", {}),
         ("gpu_pull_scalar", dict(order=G.PR_ORDER_DEGREE)),
         ("gpu_pull_adaptive", dict(layout=G.PR_LAYOUT_HOT)),                # LDS-cached contributions
         ("gpu_pull_adaptive", dict(layout=G.PR_LAYOUT_HOT, order=G.PR_ORDER_DEGREE)),
+        ("gpu_pull_adaptive", dict(layout=G.PR_LAYOUT_TILED, order=G.PR_ORDER_DEGREE, large_min=4)),  # source tiles
+        ("gpu_pull_adaptive", dict(layout=G.PR_LAYOUT_TILED, order=G.PR_ORDER_DEGREE, large_min=128)),
+        ("gpu_pull_adaptive", dict(layout=G.PR_LAYOUT_TILED, large_min=1)),            # every row with arcs tiled
+        ("gpu_pull_adaptive", dict(layout=G.PR_LAYOUT_TILED, large_min=1024)),         # almost no row tiled
         ("gpu_pull_adaptive", dict(layout=B)),                              # the reference's 16 / 256 bins
         ("gpu_pull_adaptive", dict(layout=B, small_max=1 << 30, large_min=1 << 31)),  # all lane-per-vertex
         ("gpu_pull_adaptive", dict(layout=B, small_max=0, large_min=0)),              # all workgroup-per-vertex
@@ -150,7 +154,11 @@ def test_pr_errors_are_reported():
     with pytest.raises(ValueError):
         pr.run("gpu_pull_adaptive", G.pr_options(medium_lanes=3))
     with pytest.raises(ValueError):
-        pr.run("gpu_pull_adaptive", G.pr_options(layout=3))
+        pr.run("gpu_pull_adaptive", G.pr_options(layout=G.PR_LAYOUT_TILED + 1))
+    with pytest.raises(ValueError):
+        pr.run("gpu_pull_adaptive", G.pr_options(layout=G.PR_LAYOUT_TILED, large_min=0))
+    with pytest.raises(ValueError):
+        pr.run("gpu_pull_adaptive", G.pr_options(layout=G.PR_LAYOUT_TILED, large_min=1025))
     with pytest.raises(ValueError):
         pr.run("gpu_push", G.pr_options(in_place=1))
 
