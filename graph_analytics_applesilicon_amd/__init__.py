@@ -191,6 +191,8 @@ def _optional_hip_symbols():
         "ggpu_msbfs_run": ([vp, _u32p, C.c_uint32, C.POINTER(BfsOpts), C.POINTER(MsbfsResult), C.POINTER(Level),
                             C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
         "ggpu_msbfs_fetch_dist": ([vp, _u32p], C.c_int),
+        "ggpu_bfs_run_noalloc": ([vp, C.c_uint64, C.c_uint64, vp, vp, C.c_uint32, C.POINTER(BfsOpts), vp,
+                                  C.POINTER(C.c_uint32)], C.c_int),
         "ggpu_pr_opts_default": ([C.POINTER(PrOpts)], None),
         "ggpu_pr_create": ([vp, C.c_uint64, C.c_uint64, _u32p, _u32p, _u32p, C.POINTER(vp)], C.c_int),
         "ggpu_pr_destroy": ([vp], None),
@@ -793,3 +795,20 @@ def install_policies(ctx: Optional[Context] = None) -> dict:
             policy_set(p, ctx, slot)
         done[slot] = p is not None
     return done
+
+
+NOALLOC_MAX_N = 1 << 24
+
+
+def bfs_run_noalloc(n: int, m: int, row_ptr: int, col_idx: int, source: int, dist: int,
+                    opts: Optional[BfsOpts] = None, ctx: Optional[Context] = None) -> int:
+    """ggpu_bfs_run_noalloc: one gpu_do traversal with the persistent loop over a
+    CSR already in device memory (row_ptr, col_idx, dist are device addresses,
+    e.g. a torch tensor's data_ptr()), using only the library's fixed working
+    array; returns the number of levels. No counterpart in the reference."""
+    ctx = ctx or default_context()
+    levels = C.c_uint32()
+    o = opts or bfs_options()
+    ctx.check(ctx.lib.ggpu_bfs_run_noalloc(ctx._h, n, m, C.c_void_p(row_ptr), C.c_void_p(col_idx), source,
+                                           C.byref(o), C.c_void_p(dist), C.byref(levels)))
+    return int(levels.value)

@@ -89,30 +89,28 @@ struct BfsEngine {
     LevelOut cpu_level(LoopState& st, HybridMirror& hm, const ggpu_bfs_opts& o);
 
     LoopState loop_start(uint32_t src, int variant, const ggpu_bfs_opts& o) const {
-        LoopState st{};
-        const uint32_t dg = host.deg(src);
-        st.size = 1;
-        st.total = dg;
-        st.scout = dg;
-        st.to_check = m;
-        st.unvisited_edges = m - dg;
-        st.bins[bin_of(dg, o.small_max, o.large_min)] = 1;
-        st.fmt = variant == GGPU_BFS_BINNED ? FMT_BINS : FMT_QUEUE;
-        return st;
+        return loop_start_state(host.deg(src), m, (uint32_t)variant, o.small_max, o.large_min);
     }
-    PolicyView policy_view(int variant, const ggpu_bfs_opts& o, const float* mlp) const {
-        PolicyView pv{};
-        pv.mode = o.policy;
-        pv.variant = (uint32_t)variant;
-        pv.alpha = o.alpha;
-        pv.beta = o.beta;
-        pv.actions = o.actions;
-        pv.n_actions = o.n_actions;
-        pv.mlp = mlp;
-        pv.mean_degree = n ? (double)m / (double)n : 0.0;
-        return pv;
-    }
+    PolicyView policy_view(int variant, const ggpu_bfs_opts& o, const float* mlp) const;
 };
+
+// The decision inputs of a run on a graph of n vertices and m arcs.
+inline PolicyView make_policy_view(int variant, const ggpu_bfs_opts& o, const float* mlp, uint64_t n, uint64_t m) {
+    PolicyView pv{};
+    pv.mode = o.policy;
+    pv.variant = (uint32_t)variant;
+    pv.alpha = o.alpha;
+    pv.beta = o.beta;
+    pv.actions = o.actions;
+    pv.n_actions = o.n_actions;
+    pv.mlp = mlp;
+    pv.mean_degree = n ? (double)m / (double)n : 0.0;
+    return pv;
+}
+
+inline PolicyView BfsEngine::policy_view(int variant, const ggpu_bfs_opts& o, const float* mlp) const {
+    return make_policy_view(variant, o, mlp, n, m);
+}
 
 __host__ __device__ inline ggpu_level trace_record(const LoopState& st, const LevelOut& r, char device, double ms) {
     ggpu_level L{};

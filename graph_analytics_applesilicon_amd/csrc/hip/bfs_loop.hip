@@ -31,6 +31,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "bfs_engine.hpp"
 #include "dl_protocol.hpp"
@@ -66,6 +68,7 @@ struct DlDev {
     uint32_t* abort;                // persistent: a workgroup gave up waiting
     unsigned long long* cmd;        // persistent: workgroup 0's command word (dl_protocol.hpp)
     uint32_t n_heavy, n, nb, trace_cap;
+    uint32_t heavy_min;             // degree from which a vertex is left to the heavy items (none: ~0u)
     uint64_t m;
 };
 
@@ -104,7 +107,7 @@ __device__ void dl_top_down(const DlDev& D, uint32_t level, bool lane_per_vertex
                 fw &= fw - 1;
                 b = D.rp[v];
                 e = D.rp[v + 1];
-                if (e - b >= kHeavyMin) v = 0xFFFFFFFFu;  // heavy: pushed by its items below
+                if (e - b >= D.heavy_min) v = 0xFFFFFFFFu;  // heavy: pushed by its items below
             }
             if (lane_per_vertex) {
                 if (v != 0xFFFFFFFFu)
@@ -150,7 +153,7 @@ __device__ void dl_bottom_up(const DlDev& D, uint32_t level, DlAcc& a) {
         bool found = false;
         if (v < D.n && !((seen >> lane) & 1ull)) {
             const uint32_t b = D.rp[v], e = D.rp[v + 1];
-            if (e > b && e - b < kHeavyMin) {
+            if (e > b && e - b < D.heavy_min) {
                 for (uint32_t x = b; x < e && !found; x += 4) {
                     const uint32_t c = min(4u, e - x);
                     uint32_t u[4];
@@ -787,6 +790,7 @@ void BfsEngine::device_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, s
     D.abort = R.ctl.p + kBarWords;
     D.cmd = (unsigned long long*)(R.ctl.p + kBarWords + 2);  // 8-byte aligned: kBarWords is even
     D.n_heavy = n_heavy;
+    D.heavy_min = kHeavyMin;
     D.n = n;
     D.nb = nb;
     D.trace_cap = (uint32_t)std::min<size_t>(tcap, 0xFFFFFFFFu);
@@ -865,4 +869,152 @@ void BfsEngine::device_loop(int variant, uint32_t src, const ggpu_bfs_opts& o, s
     }
 }
 
+
+// ------------------------------------------------- BFS without allocation
+// ggpu_bfs_run_noalloc: the persistent gpu_do loop over a CSR the caller holds
+// in device memory, with every working buffer in one fixed device array. The
+// array's layout, in 8-byte words (each region starts on a 256-byte boundary):
+//   vis   kNaNb + 32            visited bitmap (nb + 1 words used)
+//   fb    3 kNaNb + 32          frontier bitmaps, three slots of nb words
+//   part  2 kNaMaxGrid kDlParts per-workgroup level counters, two slots
+//   ds    kNaDsWords            DlState
+//   ctl   kNaCtlWords           arrival counters, abort flag, command word
+// No heavy-vertex list: every vertex takes the light paths (heavy_min = ~0u).
+// The counters that decide each level (found, next_edges) are exact on either
+// path, so the decisions and the distances equal ggpu_bfs_run's.
+constexpr uint32_t kNaMaxN = GGPU_NOALLOC_MAX_N;
+constexpr uint32_t kNaMaxGrid = 256;
+constexpr size_t kNaNb = kNaMaxN / 64;
+constexpr size_t kNaAlign = 32;  // words
+constexpr size_t na_round(size_t w) { return (w + kNaAlign - 1) / kNaAlign * kNaAlign; }
+constexpr size_t kNaVis = 0;
+constexpr size_t kNaFb = kNaVis + na_round(kNaNb + 1);
+constexpr size_t kNaPart = kNaFb + na_round(3 * kNaNb + 1);
+constexpr size_t kNaDs = kNaPart + na_round(2ull * kNaMaxGrid * kDlParts);
+constexpr size_t kNaDsWords = na_round((sizeof(DlState) + 7) / 8);
+constexpr size_t kNaCtl = kNaDs + kNaDsWords;
+constexpr uint32_t kNaCtlU32 = kBarWords + 16;
+constexpr size_t kNaCtlWords = na_round(kNaCtlU32 / 2);
+constexpr size_t kNaWords = kNaCtl + kNaCtlWords;
+static_assert(kNaWords * 8 == GGPU_NOALLOC_SCRATCH_BYTES, "GGPU_NOALLOC_SCRATCH_BYTES must match the layout");
+static_assert(kBarWords % 2 == 0, "the command word must be 8-byte aligned");
+
+// The array itself. Host code never uses this name as an address: its device
+// address, per GPU, comes from hipGetSymbolAddress (noalloc_setup).
+static __device__ __align__(256) uint64_t scratch[kNaWords];
+
+constexpr int kNaMaxDevices = 64;
+static uint64_t* g_na_ptr[kNaMaxDevices];   // scratch's device address on each GPU
+static std::string g_na_err[kNaMaxDevices]; // why it is missing, if it is
+static std::mutex g_na_mu;
+
+// Called by ggpu_ctx_create with `device` current: looks up the array's address
+// on that GPU once per process.
+void noalloc_setup(int device) {
+    if (device < 0 || device >= kNaMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (g_na_ptr[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        g_na_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return;
+    }
+    g_na_ptr[device] = static_cast<uint64_t*>(ptr);
+    g_na_err[device].clear();
+}
+
+// Loop state and control words of one run: the start state from the source's
+// degree (read here, from the caller's row offsets), the counters zeroed.
+__global__ void k_na_setup(DlState* ds, PolicyView pv, const uint32_t* rp, uint32_t src, uint64_t m,
+                           uint32_t small_max, uint32_t large_min, uint32_t* ctl) {
+    for (uint32_t i = threadIdx.x; i < kNaCtlU32; i += blockDim.x) ctl[i] = 0;
+    if (threadIdx.x == 0) {
+        DlState h{};
+        h.st = loop_start_state(rp[src + 1] - rp[src], m, GGPU_BFS_DO, small_max, large_min);
+        h.pv = pv;
+        *ds = h;
+    }
+}
+
+int bfs_run_noalloc(ggpu_ctx* ctx, uint64_t n, uint64_t m, const uint32_t* rp, const uint32_t* ci,
+                    uint32_t source, const ggpu_bfs_opts* opt, uint32_t* dist, uint32_t* levels) {
+    ggpu_bfs_opts o;
+    ggpu_bfs_opts_default(&o);
+    if (opt) o = *opt;
+    return guarded(ctx, [&] {
+        if (n == 0 || n > kNaMaxN)
+            throw ArgError("ggpu_bfs_run_noalloc: n = " + std::to_string(n) + " is outside 1.." +
+                           std::to_string(kNaMaxN) + " (the fixed working array's capacity)");
+        if (m >= (1ull << 32)) throw ArgError("ggpu_bfs_run_noalloc: the graph has 2^32 or more arcs");
+        if (source >= n) throw ArgError("source out of range");
+        if (!rp || !dist || (m && !ci)) throw ArgError("ggpu_bfs_run_noalloc: null device pointer");
+        if (o.policy == GGPU_POLICY_SCRIPTED)
+            throw ArgError("ggpu_bfs_run_noalloc: the scripted policy needs its actions in device memory");
+        if (o.policy > GGPU_POLICY_SCRIPTED) throw ArgError("unknown policy");
+        if (ctx->device < 0 || ctx->device >= kNaMaxDevices) throw ArgError("device index out of range");
+        uint64_t* w = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(g_na_mu);
+            w = g_na_ptr[ctx->device];
+            if (!w) throw ArgError("ggpu_bfs_run_noalloc: no working array on this GPU: " + g_na_err[ctx->device],
+                                   GGPU_EDEVICE);
+        }
+        if (!ctx->pinned || ctx->pinned_bytes < 16) throw ArgError("context has no staging memory", GGPU_ESTATE);
+        GGPU_HIP_TRY(hipSetDevice(ctx->device));
+        int per_cu = 0;
+        GGPU_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_dl_persistent, kDlThreads, 0));
+        if (per_cu < 1) throw ArgError("persistent BFS kernel does not fit on a compute unit", GGPU_EDEVICE);
+        const uint32_t grid = std::min<uint32_t>((uint32_t)ctx->cus, kNaMaxGrid);
+        const uint32_t nn = (uint32_t)n, nb = (nn + 63) / 64;
+        hipStream_t s = ctx->stream;
+
+        DlDev D{};
+        D.rp = rp;
+        D.ci = ci;
+        D.dist = dist;
+        D.vis = w + kNaVis;
+        D.fb = w + kNaFb;
+        D.part = reinterpret_cast<unsigned long long*>(w + kNaPart);
+        D.heavy = nullptr;
+        D.n_heavy = 0;
+        D.heavy_min = ~0u;
+        D.ds = reinterpret_cast<DlState*>(w + kNaDs);
+        D.trace = nullptr;
+        D.trace_cap = 0;
+        uint32_t* ctl = reinterpret_cast<uint32_t*>(w + kNaCtl);
+        D.bar = ctl;
+        D.abort = ctl + kBarWords;
+        D.cmd = reinterpret_cast<unsigned long long*>(ctl + kBarWords + 2);
+        D.n = nn;
+        D.nb = nb;
+        D.m = m;
+
+        const float* mlp = (o.policy == GGPU_POLICY_LEARNED && ctx->policy_set[GGPU_POLICY_SINGLE])
+                               ? ctx->policy_dev[GGPU_POLICY_SINGLE] : nullptr;
+        PolicyView pv = make_policy_view(GGPU_BFS_DO, o, mlp, n, m);
+        pv.actions = nullptr;
+        pv.n_actions = 0;
+        k_na_setup<<<1, 256, 0, s>>>(D.ds, pv, rp, source, m, o.small_max, o.large_min, ctl);
+        k_dl_init<<<blocks_for(std::max(nn, nb), 256, 4096), 256, 0, s>>>(D, source);
+        k_dl_seed<<<1, 1, 0, s>>>(D, source);
+        GGPU_HIP_TRY(hipGetLastError());
+        void* args[] = {&D};
+        GGPU_HIP_TRY(hipLaunchKernel((const void*)k_dl_persistent, dim3(grid), dim3(kDlThreads), args, 0, s));
+        uint32_t* out = static_cast<uint32_t*>(ctx->pinned);
+        GGPU_HIP_TRY(hipMemcpyAsync(&out[0], D.abort, 4, hipMemcpyDeviceToHost, s));
+        GGPU_HIP_TRY(hipMemcpyAsync(&out[1], &D.ds->levels, 4, hipMemcpyDeviceToHost, s));
+        GGPU_HIP_TRY(hipStreamSynchronize(s));
+        if (out[0]) throw ArgError("ggpu_bfs_run_noalloc: a wait of the persistent loop timed out", GGPU_ETIMEOUT);
+        if (levels) *levels = out[1];
+    });
+}
+
 }  // namespace ggpu
+
+extern "C" int ggpu_bfs_run_noalloc(ggpu_ctx* ctx, uint64_t n, uint64_t m, const uint32_t* d_row_ptr,
+                                    const uint32_t* d_col_idx, uint32_t source, const ggpu_bfs_opts* opt,
+                                    uint32_t* d_dist, uint32_t* levels) {
+    if (!ctx) return GGPU_EINVAL;
+    return ggpu::bfs_run_noalloc(ctx, n, m, d_row_ptr, d_col_idx, source, opt, d_dist, levels);
+}

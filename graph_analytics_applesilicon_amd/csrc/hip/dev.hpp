@@ -156,6 +156,9 @@ struct ggpu_ctx {
 namespace ggpu {
 // Pinned host buffer of at least `bytes` owned by the context.
 void* ctx_pinned(ggpu_ctx* ctx, size_t bytes);
+// With `device` current: finds the device address of the working array of
+// ggpu_bfs_run_noalloc on that GPU (once per process and GPU).
+void noalloc_setup(int device);
 // Runs f, translating exceptions into a status and ctx->err.
 template <class F>
 int guarded(ggpu_ctx* ctx, F&& f);

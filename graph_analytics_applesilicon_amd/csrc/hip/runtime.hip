@@ -48,6 +48,7 @@ int ggpu_ctx_create(int device, ggpu_ctx** out) {
         c->cus = prop.multiProcessorCount;
         c->hbm = prop.totalGlobalMem;
         GGPU_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        noalloc_setup(device);
         for (auto& p : c->policy_dev) GGPU_HIP_TRY(hipMalloc((void**)&p, GGPU_MLP_PARAMS * sizeof(float)));
         ctx_pinned(c, 4096);
     });

@@ -63,6 +63,21 @@ GGPU_HD inline uint32_t bin_of(uint32_t deg, uint32_t small_max, uint32_t large_
     return deg < small_max ? 0u : (deg < large_min ? 1u : 2u);
 }
 
+// The state before level 0: the source alone in the frontier, `src_deg` its
+// degree, m the graph's arcs.
+GGPU_HD inline LoopState loop_start_state(uint32_t src_deg, uint64_t m, uint32_t variant, uint32_t small_max,
+                                          uint32_t large_min) {
+    LoopState st{};
+    st.size = 1;
+    st.total = src_deg;
+    st.scout = src_deg;
+    st.to_check = m;
+    st.unvisited_edges = m - src_deg;
+    st.bins[bin_of(src_deg, small_max, large_min)] = 1;
+    st.fmt = variant == GGPU_BFS_BINNED ? FMT_BINS : FMT_QUEUE;
+    return st;
+}
+
 GGPU_HD inline void policy_features(uint64_t n_f, uint64_t m_f, uint64_t m_u, uint32_t level,
                                                 double mean_degree, float* f) {
     f[0] = (float)(log2(1.0 + (double)n_f) / 32.0);

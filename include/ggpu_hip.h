@@ -106,6 +106,32 @@ int ggpu_bfs_fetch_dist(ggpu_bfs* b, uint32_t* out);
 double ggpu_bfs_last_gpu_ms(const ggpu_bfs* b);
 uint32_t ggpu_bfs_last_levels(const ggpu_bfs* b);
 
+/* ------------------------------------------------ BFS without allocation
+ * No reference counterpart. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): one gpu_do traversal from `source` with the persistent device
+ * loop, over a symmetric CSR that the caller already holds in device memory
+ * (d_row_ptr: n + 1 offsets, d_col_idx: m vertex ids < n; not checked),
+ * writing the distances into the caller's d_dist (n entries, GGPU_INF if
+ * unreached); *levels (if not NULL) = the number of levels, as
+ * ggpu_bfs_last_levels reports. The distances and the per-level decisions
+ * equal ggpu_bfs_run(GGPU_BFS_DO, loop = GGPU_LOOP_PERSISTENT)'s. The call
+ * allocates and frees nothing: every working buffer (visited and frontier
+ * bitmaps, level counters, loop state, arrival counters) lives in one fixed
+ * array of GGPU_NOALLOC_SCRATCH_BYTES of device memory inside the library,
+ * which each GPU it runs on holds until the process ends; n above
+ * GGPU_NOALLOC_MAX_N is rejected (GGPU_EINVAL). Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (two host threads, two
+ * contexts, two streams) race and must not be made. The call runs on the
+ * context's stream and returns when the traversal has finished. opt (NULL =
+ * defaults): alpha, beta, small_max, large_min and the heuristic or learned
+ * policy are used, `loop` is ignored, the scripted policy is rejected. */
+#define GGPU_NOALLOC_MAX_N (1u << 24)
+#define GGPU_NOALLOC_SCRATCH_BYTES 8406528u
+int ggpu_bfs_run_noalloc(ggpu_ctx* ctx, uint64_t n, uint64_t m, const uint32_t* d_row_ptr,
+                         const uint32_t* d_col_idx, uint32_t source, const ggpu_bfs_opts* opt, uint32_t* d_dist,
+                         uint32_t* levels);
+
 /* --------------------------------------------- batched multi-source BFS
  * No reference counterpart. k sources (1 <= k <= 256) traverse together as
  * per-vertex bit masks (one bit per source, ceil(k/64) u64 words per vertex);
