@@ -17,22 +17,22 @@
 //     pair is discovered once.
 // Vertices of degree >= heavy_min are scanned / pushed as chunk-edge items
 // by whole waves (both directions), the rest lane per vertex.
-// Depths are not written during the traversal: the level slots are turned
-// into depths through bit planes (plane p = OR of F over levels whose offset
-// has bit p) and written source-major with coalesced stores, when the C
-// slots are used up and at the end. That pass also clears what it read, so
-// visit / next / F / fb are all zero between launches.
+// Depths are not written during the traversal. A launch of k_msbfs ends when
+// its C level slots are used up (saving the loop state; the next launch
+// resumes there) or when the traversal is over; then k_ms_finish turns the
+// window's level slots into depths through bit planes (plane p = OR of F over
+// levels whose offset has bit p), written source-major with coalesced stores.
+// That pass also clears what it read, so visit / next / F / fb are all zero
+// between runs. As a kernel of its own it issues every load of a vertex at
+// once (inside the persistent kernel the pass cost its level loop registers,
+// and its per-level loads ran one after another).
 // A relabeled run (batch_order DEGREE, or AUTO on a graph whose maximum degree is
 // at least 8x the mean) traverses the graph renumbered by descending degree (the
 // frontier words most arcs gather then share a few MB of lines, which stay in
 // each XCD's L2): the depth pass writes a 32-byte record per vertex (four
 // planes, a 4-bit code per source) in that numbering, and a separate kernel, k_ms_unpermute,
-// reads the record of perm[u] for every caller id u and writes u's depths.
-// Un-permute code inside the persistent kernel would cost its level loop
-// registers (scratch spills, ~10% slower levels), so a relabeled launch ends
-// when its C level slots are used up, after saving the loop state; the host
-// runs the un-permute of that window and launches the kernel again, which
-// resumes at the saved level.
+// reads the record of perm[u] for every caller id u and writes u's depths,
+// after each k_ms_finish.
 //
 // Cross-block data written inside a level goes through device-scope atomics
 // or write-through stores only (grid_sync.hpp), and every decision input
@@ -76,6 +76,8 @@ enum : uint32_t {
     kStBase,         // first level of the window whose records the un-permute reads
     kStLoopTicks,    // device clock spent in the level loop by earlier launches
     kStSetupTicks,
+    kStCnt,          // the window k_ms_finish retires: levels [kStBase, kStBase + kStCnt)
+    kStFinal,        // 1: the traversal is over (k_ms_finish also writes INF and clears visit)
     kMsState
 };
 static_assert(kMsSlots - 1 <= kCodeSkip, "depth offsets within a window stay below the two special codes");
@@ -86,6 +88,7 @@ constexpr uint32_t kPartWords = 4;
 // the bottom-up copy of col_idx an arc to such a vertex holds kMsHotBit | slot.
 constexpr uint32_t kMsHotWords = 16384;  // 128 KiB of LDS
 constexpr uint32_t kMsHotBit = 1u << 31;
+constexpr uint32_t kMsZeroWords = 64;
 
 struct MsDev {
     const uint32_t* rp;
@@ -96,13 +99,16 @@ struct MsDev {
     uint64_t* next;                // n * W, top-down push target (zero between levels)
     uint64_t* F;                   // C * n * W
     uint64_t* fb;                  // C * nb
+    const uint64_t* zero;          // kMsZeroWords zero words: the load of a lane that takes nothing
+    const uint64_t* iso;           // nb words, bit v % 64: v has no arcs or v >= n (never scanned)
+    uint64_t* done;                // nb words, bit v % 64: every source has reached v (zero between runs)
     unsigned long long* part;      // 2 * G * kPartWords
     uint32_t* dist;                // k * n, source-major
     const uint32_t* sources;       // k
     const uint4* heavy;            // {v, first edge, end edge, degree}
     const uint32_t* perm;          // relabeled runs: perm[caller id] = traversal id (else null)
     uint64_t* rec;                 // relabeled runs: n * W depth records (four planes of codes)
-    unsigned long long* state;     // kMsState words: the loop state between launches of one run
+    unsigned long long* state;     // kMsState words: the loop state between the launches of one run
     uint32_t* bar;                 // kBarWords
     uint32_t* abort;
     uint32_t* out;                 // {levels, done, actions[3], 0, setup, loop, finish clock ticks}
@@ -177,11 +183,16 @@ template <int W>
 __device__ __forceinline__ void ms_discover(const MsDev& D, uint64_t* Fn, uint64_t* fbn, uint32_t u,
                                             const uint64_t (&x)[W], MsAcc& a) {
     uint32_t pc = 0;
+    bool full = true;  // every word of u is now complete (a word this call does not touch: unknown)
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        if (!x[w]) continue;
+        if (!x[w]) {
+            full = false;
+            continue;
+        }
         const uint64_t old = atomicOr((unsigned long long*)&D.visit[(size_t)u * W + w], (unsigned long long)x[w]);
         const uint64_t fresh = x[w] & ~old;
+        full &= (old | x[w]) == valid_word<W>(D, w);
         if (fresh) {
             atomicOr((unsigned long long*)&Fn[(size_t)u * W + w], (unsigned long long)fresh);
             pc += (uint32_t)__popcll(fresh);
@@ -191,6 +202,7 @@ __device__ __forceinline__ void ms_discover(const MsDev& D, uint64_t* Fn, uint64
         const uint64_t dg = D.rp[u + 1] - D.rp[u];
         a.pairs += dg * pc;
         const uint64_t bit = 1ull << (u & 63);
+        if (full) atomicOr((unsigned long long*)&D.done[u >> 6], (unsigned long long)bit);
         const uint64_t old = atomicOr((unsigned long long*)&fbn[u >> 6], (unsigned long long)bit);
         if (!(old & bit)) {
             a.size += 1;
@@ -233,42 +245,70 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, bool 
         __syncthreads();
     }
 
-    // light vertices: lane per vertex, 64 consecutive vertices per wave
-    for (uint32_t wd = gwave(); wd < D.nb; wd += nwaves()) {
+    // light vertices: lane per vertex, 64 consecutive vertices per wave. The
+    // wave's words are gwave + i * nwaves; lane i loads the skip word of the
+    // i-th at once, and a word whose vertices are all isolated or complete is
+    // not visited (in the degree numbering the isolated vertices fill the
+    // last words)
+    const uint64_t gw = gwave(), nwv = nwaves();
+    for (uint64_t i0 = 0; gw + i0 * nwv < D.nb; i0 += kWave) {
+        const uint64_t my_wd = gw + (i0 + lane) * nwv;
+        const uint64_t my_skip = my_wd < D.nb ? (D.iso[my_wd] | D.done[my_wd]) : ~0ull;
+        uint64_t words = __ballot(my_skip != ~0ull);
+        while (words) {
+        const int jw = __ffsll((long long)words) - 1;
+        words &= words - 1;
+        const uint32_t wd = (uint32_t)(gw + (i0 + (uint64_t)jw) * nwv);
+        const uint64_t skip = __shfl(my_skip, jw, kWave);
         const uint32_t v = wd * 64 + lane;
-        bool found = false;
-        if (v < D.n) {
+        bool found = false, complete = false;
+        if (!((skip >> lane) & 1ull)) {
+            // v < n (past-the-end vertices are in iso); visit with the offsets
             const uint32_t b = D.rp[v], e = D.rp[v + 1];
+            uint64_t vis[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) vis[w] = D.visit[(size_t)v * W + w];
             const uint32_t dg = e - b;
             if (dg > 0 && dg < D.heavy_min) {
                 uint64_t need[W], acc[W];
                 bool any = false;
 #pragma unroll
                 for (int w = 0; w < W; ++w) {
-                    need[w] = ~D.visit[(size_t)v * W + w] & valid_word<W>(D, w);
+                    need[w] = ~vis[w] & valid_word<W>(D, w);
                     acc[w] = 0;
                     any |= need[w] != 0;
                 }
                 if (any) {
                     for (uint32_t x = b; x < e; x += 4) {
+                        // branch-free, so that the four arcs' loads are in flight
+                        // together: clamped arc loads, every test a select, and a
+                        // lane that takes nothing gathers from the zero words
                         const uint32_t c = min(4u, e - x);
                         uint32_t u[4];
-                        uint64_t f[4];
 #pragma unroll
-                        for (uint32_t i = 0; i < 4; ++i) u[i] = i < c ? cx[x + i] : 0u;
+                        for (uint32_t i = 0; i < 4; ++i) u[i] = cx[min(x + i, e - 1)];
+                        uint64_t fw[4];
 #pragma unroll
-                        for (uint32_t i = 0; i < 4; ++i)
-                            f[i] = i < c && !(u[i] & kMsHotBit) ? (direct ? 1ull : fb[u[i] >> 6] >> (u[i] & 63))
-                                                                : 0ull;
+                        for (uint32_t i = 0; i < 4; ++i) fw[i] = direct ? ~0ull : fb[(u[i] & ~kMsHotBit) >> 6];
+                        uint64_t g[4][W], h[4][W];
 #pragma unroll
                         for (uint32_t i = 0; i < 4; ++i) {
-                            if (i < c && (u[i] & kMsHotBit)) {
+                            const uint32_t id = u[i] & ~kMsHotBit;
+                            const bool hot = (u[i] & kMsHotBit) != 0;
+                            const bool take = i < c && !hot && ((fw[i] >> (id & 63)) & 1ull);
+                            const uint64_t* src = take ? F + (size_t)id * W : D.zero;
+                            const uint32_t slot = hot ? id : 0u;
 #pragma unroll
-                                for (int w = 0; w < W; ++w) acc[w] |= s_hot[(u[i] & ~kMsHotBit) * W + w];
-                            } else if (f[i] & 1ull) {
-#pragma unroll
-                                for (int w = 0; w < W; ++w) acc[w] |= F[(size_t)u[i] * W + w];
+                            for (int w = 0; w < W; ++w) {
+                                g[i][w] = src[w];
+                                h[i][w] = s_hot[slot * W + w];
                             }
+                        }
+#pragma unroll
+                        for (uint32_t i = 0; i < 4; ++i) {
+                            const bool hot = i < c && (u[i] & kMsHotBit);
+#pragma unroll
+                            for (int w = 0; w < W; ++w) acc[w] |= g[i][w] | (hot ? h[i][w] : 0ull);
                         }
                         a.examined += c;
                         bool all = true;
@@ -284,13 +324,19 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, bool 
                         pc += (uint32_t)__popcll(nw[w]);
                     }
                     if (pc) {
-                        // a light vertex is scanned by this lane only: its bits are new
+                        // a light vertex is scanned by this lane only and nothing
+                        // else writes its words during the level: its bits are new,
+                        // its visit word is ~need, its next-level word is zero
+                        // (write-through stores: no atomic fetches the line)
                         found = true;
+                        complete = true;
+#pragma unroll
+                        for (int w = 0; w < W; ++w) complete &= nw[w] == need[w];
 #pragma unroll
                         for (int w = 0; w < W; ++w)
                             if (nw[w]) {
-                                atomicOr((unsigned long long*)&D.visit[(size_t)v * W + w], (unsigned long long)nw[w]);
-                                atomicOr((unsigned long long*)&Fn[(size_t)v * W + w], (unsigned long long)nw[w]);
+                                store_through(&D.visit[(size_t)v * W + w], (~need[w] & valid_word<W>(D, w)) | nw[w]);
+                                store_through(&Fn[(size_t)v * W + w], nw[w]);
                             }
                         a.pairs += (uint64_t)dg * pc;
                         a.edges += dg;
@@ -299,9 +345,12 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, bool 
             }
         }
         const uint64_t mask = __ballot(found);
+        const uint64_t cmask = __ballot(complete);
         if (lane == 0 && mask) {
             atomicOr((unsigned long long*)&fbn[wd], (unsigned long long)mask);
             a.size += (uint64_t)__popcll(mask);
+        }
+        if (lane == 0 && cmask) atomicOr((unsigned long long*)&D.done[wd], (unsigned long long)cmask);
         }
     }
 
@@ -324,10 +373,13 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, bool 
             }
         }
         uint64_t todo = __ballot(want);
+        uint64_t mine[W];  // bits the wave found for this lane's item
+#pragma unroll
+        for (int w = 0; w < W; ++w) mine[w] = 0;
         while (todo) {
             const int j = __ffsll((long long)todo) - 1;
             todo &= todo - 1;
-            const uint32_t v = __shfl(item.x, j, kWave), y = __shfl(item.y, j, kWave), z = __shfl(item.z, j, kWave);
+            const uint32_t y = __shfl(item.y, j, kWave), z = __shfl(item.z, j, kWave);
             uint64_t nd[W], acc[W];
 #pragma unroll
             for (int w = 0; w < W; ++w) {
@@ -335,27 +387,37 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, bool 
                 acc[w] = 0;
             }
             for (uint32_t xb = y; xb < z; xb += 4 * kWave) {
+                // branch-free as in the light scan (z > y: the item is not empty)
                 uint32_t u[4];
-                uint64_t f[4];
                 bool in[4];
 #pragma unroll
                 for (uint32_t i = 0; i < 4; ++i) {
                     const uint32_t x = xb + i * kWave + lane;
                     in[i] = x < z;
-                    u[i] = in[i] ? cx[x] : 0u;
+                    u[i] = cx[min(x, z - 1)];
                 }
+                uint64_t fw[4];
 #pragma unroll
-                for (uint32_t i = 0; i < 4; ++i)
-                    f[i] = in[i] && !(u[i] & kMsHotBit) ? (direct ? 1ull : fb[u[i] >> 6] >> (u[i] & 63)) : 0ull;
+                for (uint32_t i = 0; i < 4; ++i) fw[i] = direct ? ~0ull : fb[(u[i] & ~kMsHotBit) >> 6];
+                uint64_t g[4][W], h[4][W];
 #pragma unroll
                 for (uint32_t i = 0; i < 4; ++i) {
-                    if (in[i] && (u[i] & kMsHotBit)) {
+                    const uint32_t id = u[i] & ~kMsHotBit;
+                    const bool hot = (u[i] & kMsHotBit) != 0;
+                    const bool take = in[i] && !hot && ((fw[i] >> (id & 63)) & 1ull);
+                    const uint64_t* src = take ? F + (size_t)id * W : D.zero;
+                    const uint32_t slot = hot ? id : 0u;
 #pragma unroll
-                        for (int w = 0; w < W; ++w) acc[w] |= s_hot[(u[i] & ~kMsHotBit) * W + w];
-                    } else if (f[i] & 1ull) {
-#pragma unroll
-                        for (int w = 0; w < W; ++w) acc[w] |= F[(size_t)u[i] * W + w];
+                    for (int w = 0; w < W; ++w) {
+                        g[i][w] = src[w];
+                        h[i][w] = s_hot[slot * W + w];
                     }
+                }
+#pragma unroll
+                for (uint32_t i = 0; i < 4; ++i) {
+                    const bool hot = in[i] && (u[i] & kMsHotBit);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) acc[w] |= g[i][w] | (hot ? h[i][w] : 0ull);
                     a.examined += in[i];
                 }
                 bool all = true;
@@ -369,17 +431,16 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, bool 
                 }
                 if (all) break;
             }
-            if (lane == 0) {
-                uint64_t x[W];
-                bool any = false;
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    x[w] = acc[w] & nd[w];
-                    any |= x[w] != 0;
-                }
-                if (any) ms_discover<W>(D, Fn, fbn, v, x, a);
-            }
+            for (int w = 0; w < W; ++w)
+                if (lane == (uint32_t)j) mine[w] = acc[w] & nd[w];
         }
+        // the round's discoveries together, each by the lane that holds the item
+        // (one set of atomic round trips per round instead of one per item)
+        bool any = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) any |= mine[w] != 0;
+        if (any) ms_discover<W>(D, Fn, fbn, item.x, mine, a);
     }
 }
 
@@ -413,9 +474,13 @@ __device__ __forceinline__ void ms_push4(const MsDev& D, uint64_t* fbn, const ui
     for (uint32_t i = 0; i < 4; ++i)
 #pragma unroll
         for (int w = 0; w < W; ++w) {
+            // both loads unconditional (a lane that takes nothing reads the zero
+            // words), so the eight are in flight together
             const bool ok = i < c && f[w];
-            vis[i][w] = ok ? D.visit[(size_t)u[i] * W + w] : ~0ull;
-            nxt[i][w] = ok ? D.next[(size_t)u[i] * W + w] : ~0ull;
+            const uint64_t pv = *(ok ? D.visit + (size_t)u[i] * W + w : D.zero);
+            const uint64_t pn = *(ok ? D.next + (size_t)u[i] * W + w : D.zero);
+            vis[i][w] = ok ? pv : ~0ull;
+            nxt[i][w] = ok ? pn : ~0ull;
         }
 #pragma unroll
     for (uint32_t i = 0; i < 4; ++i) {
@@ -442,7 +507,7 @@ __device__ __forceinline__ void ms_push_range(const MsDev& D, uint64_t* fbn, uin
 #pragma unroll
         for (uint32_t i = 0; i < 4; ++i) {
             const uint32_t y = x + i * step;
-            u[i] = y < e ? D.ci[y] : 0u;
+            u[i] = D.ci[min(y, e - 1)];  // x < e: a valid arc
             c += y < e;
         }
         ms_push4<W, kMark, kFilter>(D, fbn, u, c, f);
@@ -605,14 +670,17 @@ __device__ void ms_td_collect_all(const MsDev& D, uint32_t level, MsAcc& a) {
     for (uint64_t b = (uint64_t)gwave() * kWave; b < D.n; b += (uint64_t)nwaves() * kWave) {
         const uint32_t u = (uint32_t)b + lane;
         uint32_t pc = 0;
+        bool full = u < D.n;
         if (u < D.n) {
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 const size_t i = (size_t)u * W + w;
                 const uint64_t nx = D.next[i];
+                full &= nx != 0;
                 if (nx) {
                     const uint64_t vis = D.visit[i];
                     const uint64_t nw = nx & ~vis;
+                    full &= (vis | nw) == valid_word<W>(D, w);
                     store_through(&D.next[i], 0ull);
                     if (nw) {
                         store_through(&Fn[i], nw);
@@ -623,6 +691,8 @@ __device__ void ms_td_collect_all(const MsDev& D, uint32_t level, MsAcc& a) {
             }
         }
         const uint64_t found = __ballot(pc != 0);
+        const uint64_t cmask = __ballot(pc != 0 && full);
+        if (lane == 0 && cmask) atomicOr((unsigned long long*)&D.done[b >> 6], (unsigned long long)cmask);
         if (pc) {
             const uint64_t dg = D.rp[u + 1] - D.rp[u];
             a.pairs += dg * pc;
@@ -636,18 +706,6 @@ __device__ void ms_td_collect_all(const MsDev& D, uint32_t level, MsAcc& a) {
 }
 
 // ---------------------------------------------------------- depths, cleanup
-// Levels [base, base + cnt) are in slots (base + i) % C. Writes the depth of
-// every pair first reached in them; final: also INF for pairs never reached,
-// and visit cleared. The slots read are cleared for reuse with full-line
-// stores (masked partial-line clears cost several times more): write-through
-// inside the traversal, plain in the final pass (the launch's end writes the
-// caches back).
-template <int W>
-__device__ __forceinline__ void ms_clear(uint64_t* p, bool final) {
-    if (final) *p = 0ull;
-    else store_through(p, 0ull);
-}
-
 // Depth rows of sources s0 + b (b < ns) at one vertex: out[b * stride] =
 // base + (bit b of the four planes) where `any` has bit b, INF where
 // `unreached` has it too (a pair never reached has no plane bits). The loop
@@ -695,40 +753,55 @@ __device__ __forceinline__ void ms_write_codes(uint32_t* out, size_t stride, uin
 
 typedef unsigned long long ms_u64x2 __attribute__((ext_vector_type(2)));
 
+constexpr uint32_t kMsWin = kMsSlots - 1;  // levels one window retires at most
+
+// After each launch of k_msbfs: the window's levels [base, base + cnt) are in
+// slots (base + i) % C. Writes the depth of every pair first reached in them
+// (relabeled runs: the depth records); final: also INF for pairs never
+// reached, and visit cleared. Clears the slots it read (F where the frontier
+// bit is set: elsewhere F is already zero). A wave owns 64 consecutive
+// vertices, so their frontier words are one word per level; all of them, and
+// then every F word of the vertex's levels, are loaded before any is used.
 template <int W>
-__device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool final) {
+__global__ void __launch_bounds__(256) k_ms_finish(MsDev D) {
+    const uint32_t base = (uint32_t)D.state[kStBase], cnt = (uint32_t)D.state[kStCnt];
+    const bool final = D.state[kStFinal] != 0;
     const uint32_t lane = lane_id();
+    uint32_t sl[kMsWin];
+#pragma unroll
+    for (uint32_t i = 0; i < kMsWin; ++i) sl[i] = (base + i) % D.C;
     const uint32_t stride = gridDim.x * blockDim.x;  // a multiple of 64: lane 0 owns v % 64 == 0
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < D.n; v += stride) {
         const uint32_t wd = v >> 6;
-        uint64_t levels = 0;  // bit i: v is in the frontier of level base + i
-        for (uint32_t i = 0; i < cnt; ++i) {
-            const uint32_t s = (base + i) % D.C;
-            if ((D.fb[(size_t)s * D.nb + wd] >> (v & 63)) & 1ull) levels |= 1ull << i;
-        }
+        uint64_t fw[kMsWin];
+#pragma unroll
+        for (uint32_t i = 0; i < kMsWin; ++i) fw[i] = i < cnt ? D.fb[(size_t)sl[i] * D.nb + wd] : 0ull;
+        uint32_t levels = 0;  // bit i: v is in the frontier of level base + i
+#pragma unroll
+        for (uint32_t i = 0; i < kMsWin; ++i) levels |= (uint32_t)((fw[i] >> (v & 63)) & 1ull) << i;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
+            const size_t vw = (size_t)v * W + w;
+            uint64_t f[kMsWin];
+#pragma unroll
+            for (uint32_t i = 0; i < kMsWin; ++i)
+                f[i] = ((levels >> i) & 1u) ? D.F[(size_t)sl[i] * D.n * W + vw] : 0ull;
+            const uint64_t unreached = final ? ~D.visit[vw] : 0ull;
             uint64_t plane[kMsPlanes] = {0, 0, 0, 0}, reached = 0;
-            // uniform over the window's levels: each load is one coalesced
-            // (masked) access across the wave's 64 consecutive vertices
-            for (uint32_t i = 0; i < cnt; ++i) {
-                const uint64_t f = ((levels >> i) & 1ull) ? D.F[((size_t)((base + i) % D.C) * D.n + v) * W + w] : 0ull;
-                reached |= f;
+#pragma unroll
+            for (uint32_t i = 0; i < kMsWin; ++i) {
+                reached |= f[i];
 #pragma unroll
                 for (uint32_t p = 0; p < kMsPlanes; ++p)
-                    if ((i >> p) & 1u) plane[p] |= f;
+                    if ((i >> p) & 1u) plane[p] |= f[i];
             }
-            uint64_t unreached = 0;
-            if (final) {
-                unreached = ~D.visit[(size_t)v * W + w];
-                D.visit[(size_t)v * W + w] = 0ull;
-            }
+            if (final) D.visit[vw] = 0ull;
             const uint64_t any = reached | unreached;
             if (D.perm) {
-                // the record of (v, w), read by the next k_ms_unpermute: a pair
-                // with no plane bits gets kCodeInf if never reached, else kCodeSkip
+                // the record of (v, w), read by k_ms_unpermute: a pair with no
+                // plane bits gets kCodeInf if never reached, else kCodeSkip
                 const uint64_t hi = unreached | ~any;
-                ms_u64x2* r = (ms_u64x2*)(D.rec + ((size_t)v * W + w) * kMsRec);
+                ms_u64x2* r = (ms_u64x2*)(D.rec + vw * kMsRec);
                 r[0] = ms_u64x2{plane[0] | unreached, plane[1] | hi};
                 r[1] = ms_u64x2{plane[2] | hi, plane[3] | hi};
                 continue;
@@ -737,22 +810,24 @@ __device__ void ms_materialize(const MsDev& D, uint32_t base, uint32_t cnt, bool
             const uint32_t s0 = (uint32_t)w * 64;
             ms_write_depths(D.dist + (size_t)s0 * D.n + v, D.n, min(64u, D.k - s0), base, plane, unreached, any);
         }
-        for (uint32_t i = 0; i < cnt; ++i) {
-            const size_t slot = (base + i) % D.C;
-            // F of a vertex outside level i's frontier is already zero
-            if ((levels >> i) & 1ull) {
 #pragma unroll
-                for (int w = 0; w < W; ++w) ms_clear<W>(&D.F[(slot * D.n + v) * W + w], final);
+        for (uint32_t i = 0; i < kMsWin; ++i) {
+            if (i >= cnt) continue;
+            if ((levels >> i) & 1u) {
+#pragma unroll
+                for (int w = 0; w < W; ++w) D.F[(size_t)sl[i] * D.n * W + (size_t)v * W + w] = 0ull;
             }
             // the whole wave has read the frontier word of its 64-vertex group
-            if (lane == 0) ms_clear<W>(&D.fb[slot * D.nb + wd], final);
+            if (lane == 0) D.fb[(size_t)sl[i] * D.nb + wd] = 0ull;
         }
+        if (final && lane == 0) D.done[wd] = 0ull;
     }
 }
 
-// Relabeled runs, after the launch that retired a window: caller id u takes
-// the record of perm[u] (one random 32-byte read) and writes its depth rows
-// with the wave's 64 consecutive u, as the direct depth pass does.
+// Relabeled runs, after each k_ms_finish: caller id u takes the record of
+// perm[u] (one random 32-byte read) and writes its depth rows with the wave's
+// 64 consecutive u, as the direct depth pass does (four ids per thread with
+// 16-byte stores measured slower: 256 against 232 us on the scale-22 batch).
 template <int W>
 __global__ void __launch_bounds__(256) k_ms_unpermute(MsDev D) {
     const uint32_t base = (uint32_t)D.state[kStBase];
@@ -819,7 +894,7 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
         for (uint32_t i = threadIdx.x; i < GGPU_MLP_PARAMS; i += blockDim.x) s_mlp[i] = D.mlp[i];
     uint64_t clk[4] = {0, 0, 0, 0};
     if (blockIdx.x == 0 && threadIdx.x == 0) clk[0] = wall_clock64();
-    const bool resume = D.state && D.state[kStResume];
+    const bool resume = D.state[kStResume] != 0;
     if (!resume) {
         MsAcc acc;
         ms_seed<W>(D, acc);
@@ -892,28 +967,24 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
             __syncthreads();
             if (act == 0xFFFFFFFFu) break;
             if (level + 1 - base == D.C) {
-                // slot (level + 1) % C still holds level `base`: retire levels [base, level)
-                ms_materialize<W>(D, base, level - base, false);
-                if (D.perm) {
-                    // relabeled: the window's records are written; end the launch
-                    // and let the host run their un-permute, then resume here
-                    if (blockIdx.x == 0 && threadIdx.x == 0) {
-                        unsigned long long* st = D.state;
-                        st[kStResume] = 1;
-                        st[kStLevel] = level;
-                        st[kStAct] = act;
-                        st[kStBottomUp] = ps.bottom_up;
-                        st[kStPrevSize] = ps.prev_size;
-                        for (int i = 0; i < 3; ++i) st[kStCounts + i] = counts[i];
-                        st[kStCumPairs] = cum_pairs;
-                        st[kStBase] = base;
-                        st[kStLoopTicks] += wall_clock64() - clk[1];
-                        if (!resume) st[kStSetupTicks] = clk[1] - clk[0];
-                    }
-                    return;
+                // slot (level + 1) % C still holds level `base`: end the launch;
+                // k_ms_finish retires levels [base, level), the next launch resumes here
+                if (blockIdx.x == 0 && threadIdx.x == 0) {
+                    unsigned long long* st = D.state;
+                    st[kStResume] = 1;
+                    st[kStLevel] = level;
+                    st[kStAct] = act;
+                    st[kStBottomUp] = ps.bottom_up;
+                    st[kStPrevSize] = ps.prev_size;
+                    for (int i = 0; i < 3; ++i) st[kStCounts + i] = counts[i];
+                    st[kStCumPairs] = cum_pairs;
+                    st[kStBase] = base;
+                    st[kStCnt] = level - base;
+                    st[kStFinal] = 0;
+                    st[kStLoopTicks] += wall_clock64() - clk[1];
+                    if (!resume) st[kStSetupTicks] = clk[1] - clk[0];
                 }
-                if (!grid_barrier(D.bar, D.abort)) return;
-                base = level;
+                return;
             }
         }
         pending = 0xFFFFFFFFu;
@@ -939,27 +1010,22 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
         ms_flush(D, (level + 1) & 1, a);
         if (!grid_barrier(D.bar, D.abort)) return;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) clk[2] = wall_clock64();
-    ms_materialize<W>(D, base, level - base, true);
-    __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        clk[3] = wall_clock64();
+        clk[2] = wall_clock64();
         D.out[0] = level;
-        D.out[1] = 1;  // the traversal has ended (a relabeled launch that retires a window leaves 0)
+        D.out[1] = 1;  // the traversal has ended (a launch that fills its window leaves 0)
         D.out[2] = counts[0];
         D.out[3] = counts[1];
         D.out[4] = counts[2];
-        for (int i = 0; i < 3; ++i) D.out[6 + i] = (uint32_t)(clk[i + 1] - clk[i]);
-        if (D.state) {
-            // a resumed run: setup of the first launch, level loops of all of them
-            if (resume) {
-                D.out[6] = (uint32_t)D.state[kStSetupTicks];
-                D.out[7] += (uint32_t)D.state[kStLoopTicks];
-            }
-            D.state[kStBase] = base;
-            D.state[kStResume] = 0;  // the next run starts afresh
-            D.state[kStLoopTicks] = 0;
-        }
+        // setup of the first launch, level loops of all of them
+        unsigned long long* st = D.state;
+        D.out[6] = (uint32_t)(resume ? st[kStSetupTicks] : clk[1] - clk[0]);
+        D.out[7] = (uint32_t)(clk[2] - clk[1] + (resume ? st[kStLoopTicks] : 0ull));
+        st[kStBase] = base;
+        st[kStCnt] = level - base;
+        st[kStFinal] = 1;
+        st[kStResume] = 0;  // the next run starts afresh
+        st[kStLoopTicks] = 0;
     }
 }
 
@@ -970,6 +1036,8 @@ struct MsState {
     DBuf<uint32_t> dist, sources, ctl;
     DBuf<uint4> heavy;
     DBuf<uint32_t> cib, hot;  // bottom-up col_idx with hot slots, hot ids (for hot_w words per vertex)
+    DBuf<uint64_t> zero;      // kMsZeroWords zero words, never written
+    DBuf<uint64_t> iso, done; // skip bitmaps: isolated / past-the-end vertices (per numbering), complete ones
     uint32_t hot_w = 0, n_hot = 0;
     // the numbering heavy / cib / hot were built in; for DEGREE the renumbered
     // graph (host copy for rebuilding them, device copy to traverse), perm[caller
@@ -990,7 +1058,7 @@ struct MsState {
     uint32_t grid[5] = {0, 0, 0, 0, 0};
     bool clean = false;  // visit / next / F / fb all zero
     bool have = false;
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // batch start / end, depth pass start / end
     ~MsState() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -1013,6 +1081,15 @@ static const void* ms_kernel(uint32_t W) {
         case 2: return (const void*)k_msbfs<2>;
         case 3: return (const void*)k_msbfs<3>;
         default: return (const void*)k_msbfs<4>;
+    }
+}
+
+static const void* ms_finish_kernel(uint32_t W) {
+    switch (W) {
+        case 1: return (const void*)k_ms_finish<1>;
+        case 2: return (const void*)k_ms_finish<2>;
+        case 3: return (const void*)k_ms_finish<3>;
+        default: return (const void*)k_ms_finish<4>;
     }
 }
 
@@ -1113,10 +1190,8 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         const uint32_t n = e.n;
         const uint32_t nb = (n + 63) / 64;
         const size_t words = (size_t)n * W;
-        if (!S.ev[0]) {
-            GGPU_HIP_TRY(hipEventCreate(&S.ev[0]));
-            GGPU_HIP_TRY(hipEventCreate(&S.ev[1]));
-        }
+        if (!S.ev[0])
+            for (auto& ev : S.ev) GGPU_HIP_TRY(hipEventCreate(&ev));
         if (!S.heavy_ready) {
             std::vector<uint4> items;
             for (uint32_t v = 0; v < n; ++v) {
@@ -1129,6 +1204,12 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             S.heavy.alloc(items.size() + 1);
             if (!items.empty())
                 GGPU_HIP_TRY(hipMemcpy(S.heavy.p, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice));
+            // vertices without arcs (and the tail of the last word) are never scanned
+            std::vector<uint64_t> iso(nb, 0);
+            for (uint32_t v = 0; v < nb * 64; ++v)
+                if (v >= n || hg.deg(v) == 0) iso[v >> 6] |= 1ull << (v & 63);
+            S.iso.alloc(nb);
+            GGPU_HIP_TRY(hipMemcpy(S.iso.p, iso.data(), (size_t)nb * 8, hipMemcpyHostToDevice));
             S.heavy_ready = true;
         }
         if (S.hot_w != W) {
@@ -1168,6 +1249,7 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             const size_t avail = free_b > dist_bytes ? (free_b - dist_bytes) / 2 : 0;
             const uint32_t C = (uint32_t)std::min<size_t>(kMsSlots, std::max<size_t>(2, avail / std::max<size_t>(slot_bytes, 1)));
             S.visit.alloc(words);
+            S.done.alloc(nb);
             S.next.alloc(words);
             S.F.alloc((size_t)C * words);
             S.fb.alloc((size_t)C * nb);
@@ -1177,21 +1259,24 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         }
         if (!S.clean) {
             GGPU_HIP_TRY(hipMemsetAsync(S.visit.p, 0, words * 8, s));
+            GGPU_HIP_TRY(hipMemsetAsync(S.done.p, 0, (size_t)nb * 8, s));
             GGPU_HIP_TRY(hipMemsetAsync(S.next.p, 0, words * 8, s));
             GGPU_HIP_TRY(hipMemsetAsync(S.F.p, 0, (size_t)S.C * words * 8, s));
             GGPU_HIP_TRY(hipMemsetAsync(S.fb.p, 0, (size_t)S.C * nb * 8, s));
             if (S.st.p) GGPU_HIP_TRY(hipMemsetAsync(S.st.p, 0, kMsState * 8, s));
             S.clean = true;
         }
+        if (!S.zero.p) {
+            S.zero.alloc(kMsZeroWords);
+            GGPU_HIP_TRY(hipMemsetAsync(S.zero.p, 0, kMsZeroWords * 8, s));
+        }
         const uint32_t G = ms_grid(ctx, S, W);
         S.part.alloc((size_t)2 * G * kPartWords);
         S.dist.alloc((size_t)k * n);
-        if (relabel) {
-            S.rec.alloc(words * kMsRec);
-            if (!S.st.p) {
-                S.st.alloc(kMsState);
-                GGPU_HIP_TRY(hipMemsetAsync(S.st.p, 0, kMsState * 8, s));
-            }
+        if (relabel) S.rec.alloc(words * kMsRec);
+        if (!S.st.p) {
+            S.st.alloc(kMsState);
+            GGPU_HIP_TRY(hipMemsetAsync(S.st.p, 0, kMsState * 8, s));
         }
         S.k = k;
         S.sources.alloc(k);
@@ -1216,7 +1301,7 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         D.ci = relabel ? S.ci2.p : e.ci.p;
         D.perm = relabel ? S.perm.p : nullptr;
         D.rec = relabel ? S.rec.p : nullptr;
-        D.state = relabel ? S.st.p : nullptr;
+        D.state = S.st.p;
         D.cib = S.cib.p;
         D.hot = S.hot.p;
         D.n_hot = S.n_hot;
@@ -1225,6 +1310,9 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         D.next = S.next.p;
         D.F = S.F.p;
         D.fb = S.fb.p;
+        D.zero = S.zero.p;
+        D.iso = S.iso.p;
+        D.done = S.done.p;
         D.part = S.part.p;
         D.dist = S.dist.p;
         D.sources = S.sources.p;
@@ -1258,19 +1346,26 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         GGPU_HIP_TRY(hipEventRecord(S.ev[0], s));
         // plain launches, one workgroup per CU (the LDS footprint allows no second):
         // the barrier is the kernel's own, bounded (see the single-source loop).
-        // A relabeled run takes one launch per window of C levels, each followed
-        // by the un-permute of that window's depth records.
+        // One launch per window of C levels, each followed by the depth pass of
+        // that window and, for a relabeled run, the un-permute of its records.
         uint32_t* ctl = pin + 256;
+        double finish_ms = 0;
         for (uint32_t launch = 0;; ++launch) {
             if (launch) GGPU_HIP_TRY(hipMemsetAsync(S.ctl.p, 0, kBarWords * sizeof(uint32_t), s));
             GGPU_HIP_TRY(hipLaunchKernel(ms_kernel(W), dim3(G), dim3(kMsThreads), args, 0, s));
+            GGPU_HIP_TRY(hipEventRecord(S.ev[2], s));
+            GGPU_HIP_TRY(hipLaunchKernel(ms_finish_kernel(W), dim3(ms_unpermute_grid(ctx, n)), dim3(256), args, 0, s));
             if (relabel)
                 GGPU_HIP_TRY(hipLaunchKernel(ms_unpermute_kernel(W), dim3(ms_unpermute_grid(ctx, n)), dim3(256),
                                              args, 0, s));
+            GGPU_HIP_TRY(hipEventRecord(S.ev[3], s));
             GGPU_HIP_TRY(hipEventRecord(S.ev[1], s));  // the last launch's record is the one timed
             GGPU_HIP_TRY(hipMemcpyAsync(ctl, S.ctl.p + kBarWords, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             GGPU_HIP_TRY(hipStreamSynchronize(s));
             if (ctl[0]) throw ArgError("batched BFS: grid barrier timed out", GGPU_ETIMEOUT);
+            float fms = 0;
+            GGPU_HIP_TRY(hipEventElapsedTime(&fms, S.ev[2], S.ev[3]));
+            finish_ms += fms;
             if (ctl[5]) break;  // out[1]: the traversal has ended
         }
 
@@ -1289,7 +1384,7 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             res->level_actions[2] = ctl[8];
             res->setup_ms = ctl[10] * D.tick_ms;
             res->levels_ms = ctl[11] * D.tick_ms;
-            res->finish_ms = ctl[12] * D.tick_ms;
+            res->finish_ms = finish_ms;
         }
         if (trace_len) *trace_len = levels;
         if (trace && trace_cap) {

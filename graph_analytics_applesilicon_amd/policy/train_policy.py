@@ -7,12 +7,14 @@ and 10 seeded grids (largest component of a 4-neighbour grid, each edge kept
 with probability 0.6-0.9). For every graph the level recorders time each
 level under all three actions (TD_EDGE, TD_BINNED, BU):
   * batched: the 64 pickSources sources (seed = graph seed) in one batch;
-  * single source: gpu_do from 4 of those sources.
+  * single source: gpu_do from 4 of those sources (host loop).
 Targets are ln(ms). Grids contribute thousands of nearly identical levels,
 so at most 256 evenly spaced levels per recording are kept. Each slot's MLP
 is trained by the library's own GPU kernels (full-batch Adam) from a seeded
 He initialisation; a held-out set of 6 more graphs reports regret = time of
-the chosen actions / time of the best actions, next to the heuristic's.
+the chosen actions / time of the best actions, and the end-to-end time of the
+learned policy against the GAP heuristic: the 64-source batch, and gpu_do
+from 4 sources under the host and the persistent loops.
 
 Usage (needs the GPU):
     python -m graph_analytics_applesilicon_amd.policy.train_policy --out DIR [--quick]
@@ -168,6 +170,27 @@ def main():
         log(f"held-out {kind} {spec}: heuristic {ms[G.POLICY_HEURISTIC]:.3f} ms, learned "
             f"{ms[G.POLICY_LEARNED]:.3f} ms")
     report["held_out_batched_ms"] = rows
+
+    # end-to-end single-source gpu_do on the held-out graphs, host and persistent loops
+    G.policy_set(np.fromfile(os.path.join(args.out, "single_source.f32"), dtype=np.float32), ctx, G.POLICY_SINGLE)
+    rows = []
+    for kind, spec in held:
+        g = make_graph(kind, spec)
+        bfs = G.GpuBfs(g)
+        src = [int(s) for s in g.pick_sources(64, spec["seed"])[:4]]
+        row = {"graph": f"{kind} {spec}"}
+        for loop, lname in ((G.LOOP_HOST, "host"), (G.LOOP_PERSISTENT, "persistent")):
+            for pol, pname in ((G.POLICY_HEURISTIC, "heuristic"), (G.POLICY_LEARNED, "learned")):
+                o = G.bfs_options(policy=pol, loop=loop)
+                tot = 0.0
+                for s in src:
+                    bfs.run("gpu_do", s, o, trace=False, fetch=False)
+                    tot += float(np.median([bfs.run("gpu_do", s, o, trace=False, fetch=False).gpu_ms
+                                            for _ in range(3)]))
+                row[f"{lname}_{pname}_ms"] = tot
+        rows.append(row)
+        log(f"held-out single {kind} {spec}: " + ", ".join(f"{k} {v:.3f}" for k, v in row.items() if k != "graph"))
+    report["held_out_single_ms"] = rows
     with open(os.path.join(args.out, "train_report.json"), "w") as f:
         json.dump(report, f, indent=1)
     log(f"done in {time.time() - t0:.1f}s")

@@ -143,8 +143,8 @@ typedef struct ggpu_msbfs_result {
     uint64_t level_actions[3];/* levels run per action */
     double setup_ms;          /* device clock: launch start -> level 0 ready */
     double levels_ms;         /* device clock: the level loop (summed over a renumbered run's launches) */
-    double finish_ms;         /* device clock: depth pass and state cleared (block 0's view); a
-                                 renumbered run's un-permute kernel is in gpu_ms only */
+    double finish_ms;         /* events: the depth pass kernels (depths, or depth records and their
+                                 un-permute for a renumbered run) and state cleared, over all windows */
 } ggpu_msbfs_result;
 /* opt->policy picks the per-level action (HEURISTIC: GAP rule on batch counts;
  * LEARNED: the MLP; SCRIPTED: opt->actions). */
