@@ -13,9 +13,11 @@ the plain per-source gpu_do rate, PageRank time to L1 < 1e-4 against the FP64
 oracle, the roofline of the dominant kernel (HIP events on the library's
 stream) and the reference's own cpu_do / cpu_omp timed on this host.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-pagerank] [--only bfs|pagerank]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-pagerank]
+                       [--only bfs|pagerank|loops|ppr|prw]
        python bench.py --copy-calibration   (a plain device copy of 1 GiB, three times: the
                                              reference kernel for the FETCH_SIZE / WRITE_SIZE scale)
+       python bench.py --gather-calibration (random 128-byte and 8-byte gathers of known size)
 The roofline's `traffic` (HBM bytes per launch) is read from
 profiles/hbm_traffic.json, which holds the rocprofv3 FETCH_SIZE and WRITE_SIZE
 readings of the dominant kernels scaled by the copy calibration.
@@ -60,6 +62,34 @@ def copy_calibration():
     print(json.dumps({"copy_bytes": COPY_BYTES, "launches": 3}), flush=True)
 
 
+GATHER_ROWS = 1 << 22
+
+
+def gather_calibration():
+    """Random gathers of known size from a 1 GiB table, three launches each
+    (torch's index_select, seeded indices): GATHER_ROWS rows of 128 bytes
+    (a whole L2 line each), then GATHER_ROWS single 8-byte elements. The
+    counters' reading per launch against the bytes the rows are, and what an
+    8-byte random gather costs as the counters see it."""
+    import torch
+    table = torch.ones((1 << 30) // 4, dtype=torch.float32, device="cuda")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    rows128 = table.view(-1, 32)
+    idx = torch.randint(0, rows128.shape[0], (GATHER_ROWS,), device="cuda", generator=gen)
+    for _ in range(3):
+        out = rows128.index_select(0, idx)
+    torch.cuda.synchronize()
+    elems8 = table.view(torch.int64)
+    idx8 = torch.randint(0, elems8.shape[0], (GATHER_ROWS,), device="cuda", generator=gen)
+    for _ in range(3):
+        out8 = elems8.index_select(0, idx8)
+    torch.cuda.synchronize()
+    del out, out8
+    print(json.dumps({"gather_rows": GATHER_ROWS, "row_bytes": [128, 8], "table_bytes": 1 << 30,
+                      "launches_each": 3}), flush=True)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -76,11 +106,15 @@ def main():
                     help="graph of the PageRank part (default: the scale-23 heavy-skew RMAT)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", default="auto", choices=["auto", "batched", "sequential"])
-    ap.add_argument("--only", default="all", choices=["all", "bfs", "pagerank", "loops", "ppr"],
+    ap.add_argument("--only", default="all", choices=["all", "bfs", "pagerank", "loops", "ppr", "prw"],
                     help="profiling runs: just one part (the JSON line then carries that part)")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--gather-calibration", action="store_true",
+                    help="random 128-byte and 8-byte gathers from a 1 GiB table (counter calibration runs)")
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 PageRank reference (profiling runs)")
     ap.add_argument("--no-loops", action="store_true", help="skip the level-loop comparison on the grid graph")
+    ap.add_argument("--no-pr-workloads", action="store_true",
+                    help="skip PageRank ms/iteration on the small RMAT, scale-22 RMAT and grid graphs")
     ap.add_argument("--no-ppr", action="store_true", help="skip the personalized-PageRank block")
     ap.add_argument("--ppr-seeds", type=int, default=64, help="columns of the personalized-PageRank block")
     ap.add_argument("--ppr-assign", default=None, metavar="ORDER:LARGE_MIN:LANES:TG",
@@ -88,6 +122,9 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.gather_calibration:
+        gather_calibration()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build runs on a single "
@@ -99,7 +136,7 @@ def main():
 
     ctx = G.default_context()
     name, cus, _ = ctx.info()
-    bfs_res = pr_res = cpu = loops = ppr = None
+    bfs_res = pr_res = cpu = loops = ppr = prw = None
     if args.only in ("all", "bfs"):
         bfs_res = bfs_part(G, O, args, ctx)
     if args.only == "loops" or (args.only == "all" and not args.no_loops):
@@ -108,6 +145,8 @@ def main():
         pg = G.Graph.named(args.pr_graph)
         pr_res = pagerank_part(G, O, pg, ctx, args.pr_graph, with_ref=not args.no_ref)
         pr_res["graph"] = args.pr_graph
+    if args.only == "prw" or (args.only == "all" and not args.no_pr_workloads):
+        prw = pr_workloads_part(G, O, ctx)
     if args.only == "ppr" or (args.only == "all" and not args.no_ppr):
         pg = pg if pr_res is not None else G.Graph.named(args.pr_graph)
         ppr = ppr_part(G, pg, ctx, args.pr_graph, args.ppr_seeds, assign=args.ppr_assign)
@@ -137,6 +176,14 @@ def main():
                        "path": bfs_res["path"], "device": name, "cus": cus},
             "roofline": bfs_res.pop("roofline"),
             "cpu_baseline": cpu,
+            # beside the headline: the same batch traversed in the caller's numbering
+            # (no renumbered copy, no un-permute), and the first call, which builds the
+            # renumbered copy of the graph and is not a timed step
+            "batch_caller_numbering": {"ms": bfs_res.get("given_order_batched_ms"),
+                                       "gteps": (bfs_res["sources"] * bfs_res["comp_edges"]
+                                                 / (bfs_res["given_order_batched_ms"] / 1e3) / 1e9
+                                                 if bfs_res.get("given_order_batched_ms") else None)},
+            "first_call_ms": bfs_res.get("first_call_ms"),
         }
         for k in ("value", "ms_per_step", "n", "m", "sources", "path"):
             bfs_res.pop(k, None)
@@ -148,6 +195,8 @@ def main():
             line["pagerank"] = pr_res
         if ppr is not None:
             line["ppr"] = ppr
+        if prw is not None:
+            line["pagerank_workloads"] = prw
     elif ppr is not None and pr_res is None:
         line = {"metric": f"personalized PageRank block ms per sweep ({args.ppr_seeds} seeds)",
                 "value": ppr["ms_per_sweep"], "unit": "ms", "n_gpus": 1, "steps": ppr["sweeps"], "warmup": 1,
@@ -155,6 +204,13 @@ def main():
                 "vs_baseline": None, "dtype": "f32", "data": "synthetic",
                 "config": {"workload": ppr["workload"], "device": name}, "roofline": ppr.pop("roofline"),
                 "cpu_baseline": None, "ppr": ppr}
+    elif pr_res is None and loops is None and prw is not None:
+        c2 = prw["rmat22_c2"]["gpu_pull_adaptive"]
+        line = {"metric": "PageRank ms per iteration (gpu_pull_adaptive, rmat22_c2)", "value": c2["ms_per_iteration"],
+                "unit": "ms", "n_gpus": 1, "steps": 20, "warmup": 1, "ms_per_step": c2["ms_per_iteration"],
+                "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                "data": "synthetic", "config": {"workload": "rmat16_c1, rmat22_c2, grid_c3", "device": name},
+                "roofline": None, "cpu_baseline": None, "pagerank_workloads": prw}
     elif pr_res is None:
         line = {"metric": "BFS us per level on the grid graph (gpu_do)", "value": loops["persistent"]["us_per_level"],
                 "unit": "us/level", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -204,6 +260,7 @@ def bfs_part(G, O, args, ctx):
         bad += int(not np.array_equal(d, O.bfs_serial(rp, ci, int(s))))
     out["plain_gpu_do"] = {"gteps": plain_gteps, "ms_per_batch": plain_s * 1e3,
                            "bit_exact_sources_checked": 4, "mismatches": bad}
+    out["single_source"] = single_source_part(G, bfs, sources, comp_edges, n, m, ctx)
 
     # ---------------------------------------------------------- timed steps
     batched = args.mode in ("auto", "batched")
@@ -274,6 +331,85 @@ def bfs_part(G, O, args, ctx):
                 "traffic": measured_traffic("k_msbfs<1>+k_ms_unpermute<1>", f"{args.graph}/k{len(sources)}"),
                 "bytes_per_launch": bytes_batch, "avg_launch_ms": avg_ms}
     out["roofline"] = roof
+    return out
+
+
+def single_source_part(G, bfs, sources, comp_edges, n, m, ctx, trials=3):
+    """gpu_do from every source, host loop against the persistent loop, heuristic
+    against the learned single-source policy: per source one warm-up and
+    `trials` timed runs (wall clock around the call, depths left on the
+    device), the median; GTEPS as the harmonic mean over sources
+    (src/bench/main.cpp:308-370). The roofline counts one pass over the CSR
+    and the depth array per traversal, 4(n+1) + 4m + 4n bytes."""
+    G.install_policies(ctx)
+    b_bfs = 4 * (n + 1) + 4 * m + 4 * n
+    res = {"bytes_per_traversal": b_bfs, "sources": len(sources)}
+    for lname, loop in (("host", G.LOOP_HOST), ("persistent", G.LOOP_PERSISTENT)):
+        for pname, pol in (("heuristic", G.POLICY_HEURISTIC), ("learned", G.POLICY_LEARNED)):
+            o = G.bfs_options(loop=loop, policy=pol, time_levels=0)
+            def once(s):
+                t = time.perf_counter()
+                bfs.run("gpu_do", int(s), o, trace=False, fetch=False)
+                return (time.perf_counter() - t) * 1e3
+
+            med = []
+            for s in sources:
+                once(s)
+                med.append(float(np.median([once(s) for _ in range(trials)])))
+            mean_ms = float(np.mean(med))
+            ach = b_bfs / (mean_ms / 1e3) / 1e9
+            res[f"{lname}_{pname}"] = {"gteps": comp_edges / (mean_ms / 1e3) / 1e9, "mean_ms": mean_ms,
+                                       "median_ms": float(np.median(med)), "achieved_gbs": ach,
+                                       "frac": ach / HBM_PEAK_GBS}
+            log(f"  gpu_do {lname} loop, {pname}: {mean_ms:.3f} ms per traversal, "
+                f"{res[f'{lname}_{pname}']['gteps']:.1f} GTEPS, {ach / HBM_PEAK_GBS * 100:.1f}% of HBM")
+    return res
+
+
+def pr_workloads_part(G, O, ctx, iters=20, trials=3):
+    """PageRank ms per iteration on the small RMAT, the scale-22 RMAT and the grid:
+    gpu_pull_adaptive (in-degree bins) with the default options and with the
+    autotuner's layout / order / bins, and gpu_pull_scalar (thread per vertex),
+    `iters` fixed sweeps per run (HIP events, median of `trials`), as a fraction
+    of HBM peak by 4(n+1) + 8m + 4n bytes per iteration; beside the reference's
+    cpu_omp (pagerankOmp, OpenMP on this host's cores) per iteration."""
+    rlib = O.ref_lib()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    out = {}
+    for name in ("rmat16_c1", "rmat22_c2", "grid_c3"):
+        g = G.Graph.named(name)
+        n, m = g.n, g.m
+        od = g.degrees()
+        pr = G.GpuPageRank(g, od, ctx)  # undirected: the graph is its own transpose
+        b_pr = 4 * (n + 1) + 8 * m + 4 * n
+        ent = {"n": n, "m": m, "bytes_per_iteration": b_pr, "iterations_per_run": iters}
+        tuned = pr.autotune()
+        tuned.iters, tuned.tol = iters, 0.0
+        for key, variant, o in (("gpu_pull_adaptive", "gpu_pull_adaptive", G.pr_options(iters=iters)),
+                                ("gpu_pull_adaptive_tuned", "gpu_pull_adaptive", tuned),
+                                ("gpu_pull_scalar", "gpu_pull_scalar", G.pr_options(iters=iters))):
+            pr.run(variant, o)
+            per = float(np.median([pr.run(variant, o).gpu_ms for _ in range(trials)])) / iters
+            ach = b_pr / (per / 1e3) / 1e9
+            ent[key] = {"ms_per_iteration": per, "achieved_gbs": ach, "frac": ach / HBM_PEAK_GBS}
+        ent["tuned_options"] = {"layout": int(tuned.layout), "order": int(tuned.order),
+                                "small_max": int(tuned.small_max), "large_min": int(tuned.large_min)}
+        if rlib is not None:
+            O.ref_pagerank(g.row_ptr, g.col_idx, od, 0.85, 1, 0.0, parallel=True)
+            t1 = time.perf_counter()
+            O.ref_pagerank(g.row_ptr, g.col_idx, od, 0.85, 5, 0.0, parallel=True)
+            ent["cpu_omp"] = {"kind": "reference", "cores": cores,
+                              "ms_per_iteration": (time.perf_counter() - t1) / 5 * 1e3}
+        out[name] = ent
+        log(f"  pagerank {name}: adaptive {ent['gpu_pull_adaptive']['ms_per_iteration'] * 1e3:.1f} us/it "
+            f"({ent['gpu_pull_adaptive']['frac'] * 100:.1f}%), tuned "
+            f"{ent['gpu_pull_adaptive_tuned']['ms_per_iteration'] * 1e3:.1f} us/it "
+            f"({ent['gpu_pull_adaptive_tuned']['frac'] * 100:.1f}%, {ent['tuned_options']}), scalar "
+            f"{ent['gpu_pull_scalar']['ms_per_iteration'] * 1e3:.1f} us/it"
+            + (f", cpu_omp {ent['cpu_omp']['ms_per_iteration']:.2f} ms/it" if "cpu_omp" in ent else ""))
     return out
 
 
@@ -445,14 +581,18 @@ def ppr_part(G, g, ctx, g_name, B=64, sweeps=10, assign=None):
     seed_min = float(blk[seeds, np.arange(B)].min())
     del blk
     # rows of the in-CSR, the lists, out-degrees, one row gather of 4B bytes per
-    # in-edge, rank and contribution rows written
+    # in-edge, rank and contribution rows written; beside it the per-iteration
+    # count without the out-degree read and the contribution rows,
+    # 4(n+1) + 4m + 4Bm + 4Bn
     bytes_sweep = 4 * (n + 1) + 4 * m + 4 * n + 4 * B * m + 8 * B * n
+    bytes_min = 4 * (n + 1) + 4 * m + 4 * B * m + 4 * B * n
     ach = bytes_sweep / (per / 1e3) / 1e9
     res = {"workload": f"{g_name}: RMAT scale 23, ef 16, a/b/c .60/.17/.17; {B} pickSources seeds (seed 5)",
            "seeds": B, "sweeps": sweeps, "ms_per_sweep": per, "gpu_ms_2x_sweeps": r2.gpu_ms,
            "autotune_s": tune_s, "assignment": {"order": "degree" if o.order == G.PR_ORDER_DEGREE else "given",
                                                "large_min": o.large_min, "medium_lanes": o.medium_lanes, "tg": o.tg},
            "max_column_mass_error": mass_err, "min_seed_rank": seed_min,
+           "bytes_4Bn_formula": bytes_min, "frac_4Bn_formula": bytes_min / (per / 1e3) / 1e9 / HBM_PEAK_GBS,
            "roofline": {"kernel": "k_ppr_chunks + k_ppr_sweep + k_ppr_reduce per sweep", "bound": "hbm",
                         "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                         "traffic": measured_traffic("k_ppr_chunks+k_ppr_sweep", g_name), "bytes_per_launch": bytes_sweep,

@@ -168,27 +168,67 @@ def test_learned_weights_keep_depths_exact_and_follow_the_forward_kernel(seed):
         G.policy_clear(ctx, G.POLICY_SINGLE)
 
 
+def _median_ms(fn, reps=5):
+    fn()
+    return float(np.median([fn() for _ in range(reps)]))
+
+
 def test_trained_batched_policy_gains_on_held_out_graphs():
     """The committed batched weights (trained by policy/train_policy.py on 50
     seeded graphs) against the GAP heuristic on graphs outside the training
-    seeds: never slower overall, depths still exact."""
+    seeds: at least 5% faster on the RMAT graph and 10% faster on the grid and
+    in total, depths still exact. The comparator is the reference's rule
+    (src/cpu/bfs_cpu.cpp:227-283) on batch counts."""
     p = G.load_policy_file(G.POLICY_BATCHED)
     assert p is not None, "graph_analytics_applesilicon_amd/policy/batched.f32 missing"
     ctx = G.default_context()
     G.policy_set(p, ctx, G.POLICY_BATCHED)
     try:
-        graphs = [G.Graph.rmat(18, 16, 0.49, 0.23, 0.23, seed=7001).permute(G.random_perm(1 << 18, 7001)),
-                  G.Graph.grid(256, 512, 0.7, 7002).largest_component()[0]]
+        graphs = [("rmat18", G.Graph.rmat(18, 16, 0.49, 0.23, 0.23, seed=7001).permute(G.random_perm(1 << 18, 7001)),
+                   0.95),
+                  ("grid256x512", G.Graph.grid(256, 512, 0.7, 7002).largest_component()[0], 0.90)]
         total = {G.POLICY_HEURISTIC: 0.0, G.POLICY_LEARNED: 0.0}
-        for g in graphs:
+        for name, g, bound in graphs:
             bfs = G.GpuBfs(g, ctx)
             src = np.array(g.pick_sources(64, 7), dtype=np.uint32)
+            ms = {}
             for pol in total:
                 o = G.bfs_options(policy=pol)
                 dist, _, _ = bfs.run_batch(src, o)
                 for i in (0, 31, 63):
                     assert np.array_equal(dist[i], O.bfs_serial(g.row_ptr, g.col_idx, int(src[i])))
-                total[pol] += float(np.median([bfs.run_batch(src, o, fetch=False)[1].gpu_ms for _ in range(5)]))
-        assert total[G.POLICY_LEARNED] <= 1.05 * total[G.POLICY_HEURISTIC], total
+                ms[pol] = _median_ms(lambda: bfs.run_batch(src, o, fetch=False)[1].gpu_ms)
+                total[pol] += ms[pol]
+            print(f"{name}: heuristic {ms[G.POLICY_HEURISTIC]:.3f} ms, learned {ms[G.POLICY_LEARNED]:.3f} ms")
+            assert ms[G.POLICY_LEARNED] <= bound * ms[G.POLICY_HEURISTIC], (name, ms)
+        assert total[G.POLICY_LEARNED] <= 0.9 * total[G.POLICY_HEURISTIC], total
     finally:
         G.policy_clear(ctx, G.POLICY_BATCHED)
+
+
+def test_trained_single_source_policy_gains_on_a_held_out_grid_with_the_host_loop():
+    """The single-source weights are trained on per-level times of the host
+    loop, and they beat the GAP heuristic there on high-diameter graphs (at
+    least 10% on a held-out grid, depths exact). They do not on RMAT graphs or
+    under the persistent loop, where the single-workgroup levels change what a
+    level costs; the heuristic stays the default for single sources."""
+    p = G.load_policy_file(G.POLICY_SINGLE)
+    assert p is not None, "graph_analytics_applesilicon_amd/policy/single_source.f32 missing"
+    ctx = G.default_context()
+    G.policy_set(p, ctx, G.POLICY_SINGLE)
+    try:
+        g = G.Graph.grid(384, 768, 0.7, 7003).largest_component()[0]
+        bfs = G.GpuBfs(g, ctx)
+        src = [int(s) for s in g.pick_sources(2, 7)]
+        ms = {}
+        for pol in (G.POLICY_HEURISTIC, G.POLICY_LEARNED):
+            o = G.bfs_options(policy=pol, loop=G.LOOP_HOST)
+            ms[pol] = 0.0
+            for s in src:
+                r = bfs.run("gpu_do", s, o, trace=False)
+                assert np.array_equal(r.dist, O.bfs_serial(g.row_ptr, g.col_idx, s))
+                ms[pol] += _median_ms(lambda: bfs.run("gpu_do", s, o, trace=False, fetch=False).gpu_ms, reps=3)
+        print(f"grid384x768 host loop: heuristic {ms[G.POLICY_HEURISTIC]:.2f} ms, learned {ms[G.POLICY_LEARNED]:.2f} ms")
+        assert ms[G.POLICY_LEARNED] <= 0.9 * ms[G.POLICY_HEURISTIC], ms
+    finally:
+        G.policy_clear(ctx, G.POLICY_SINGLE)
