@@ -191,6 +191,9 @@ def _optional_hip_symbols():
         "ggpu_msbfs_run": ([vp, _u32p, C.c_uint32, C.POINTER(BfsOpts), C.POINTER(MsbfsResult), C.POINTER(Level),
                             C.c_size_t, C.POINTER(C.c_size_t)], C.c_int),
         "ggpu_msbfs_fetch_dist": ([vp, _u32p], C.c_int),
+        "ggpu_test_scan": ([vp, C.c_uint32, _u32p, _u32p, C.c_uint32], C.c_int),
+        "ggpu_test_frontier_convert": ([vp, _u32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64), _u32p, _u32p,
+                                        C.POINTER(C.c_uint32)], C.c_int),
         "ggpu_bfs_run_noalloc": ([vp, C.c_uint64, C.c_uint64, vp, vp, C.c_uint32, C.POINTER(BfsOpts), vp,
                                   C.POINTER(C.c_uint32)], C.c_int),
         "ggpu_pr_opts_default": ([C.POINTER(PrOpts)], None),

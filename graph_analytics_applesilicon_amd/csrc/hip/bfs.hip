@@ -613,3 +613,66 @@ uint32_t ggpu_bfs_last_levels(const ggpu_bfs* b) { return b ? b->eng->last_level
 
 // Internal accessor for the other translation units of this library.
 ggpu::BfsEngine* ggpu_bfs_engine(ggpu_bfs* b) { return b ? b->eng : nullptr; }
+
+// ------------------------------------------------------------- test hooks
+// the level-loop primitives on their own, for the tests
+#include "ggpu_hip_testing.h"
+
+extern "C" {
+
+int ggpu_test_scan(ggpu_ctx* ctx, uint32_t n, const uint32_t* in, uint32_t* out, uint32_t max_blocks) {
+    if (!ctx || (n && (!in || !out))) return GGPU_EINVAL;
+    return guarded(ctx, [&] {
+        GGPU_HIP_TRY(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        const uint32_t span = 256 * kScanItems;
+        const uint32_t ntiles = (uint32_t)(((uint64_t)n + span - 1) / span);
+        const uint32_t cap = max_blocks ? max_blocks : 4096;  // the engine's grid cap
+        DBuf<uint32_t> v, tile;
+        v.alloc(n ? n : 1);
+        tile.alloc(ntiles ? ntiles : 1);
+        if (n) GGPU_HIP_TRY(hipMemcpyAsync(v.p, in, 4ull * n, hipMemcpyHostToDevice, s));
+        if (ntiles) {
+            // as the edge-balanced top-down launches it: 256-thread blocks, one tile per block
+            k_scan_tiles<<<blocks_for(ntiles, 1, cap), 256, 0, s>>>(v.p, n, tile.p);
+            k_scan_apply<<<blocks_for(ntiles, 1, cap), 256, 0, s>>>(v.p, n, tile.p);
+            GGPU_HIP_TRY(hipGetLastError());
+        }
+        if (n) GGPU_HIP_TRY(hipMemcpyAsync(out, v.p, 4ull * n, hipMemcpyDeviceToHost, s));
+        GGPU_HIP_TRY(hipStreamSynchronize(s));
+    });
+}
+
+int ggpu_test_frontier_convert(ggpu_bfs* b, const uint32_t* dist, uint32_t level, uint32_t tg, uint64_t* bitmap,
+                               uint32_t* queue, uint32_t* degrees, uint32_t* count) {
+    if (!b || !dist || !bitmap || !queue || !degrees || !count) return GGPU_EINVAL;
+    return guarded(b->ctx, [&] {
+        BfsEngine& e = *b->eng;
+        check_tg(tg);
+        GGPU_HIP_TRY(hipSetDevice(e.ctx->device));
+        hipStream_t s = e.ctx->stream;
+        e.ensure_edge_buffers();
+        e.ensure_bitmaps();
+        BfsDev d = e.dev();
+        if (e.n) GGPU_HIP_TRY(hipMemcpyAsync(d.dist, dist, 4ull * e.n, hipMemcpyHostToDevice, s));
+        // the bitmap starts all ones (the conversion must write every word), the counter at zero
+        GGPU_HIP_TRY(hipMemsetAsync(d.bm[0], 0xFF, 8ull * std::max(e.nw, 1u), s));
+        GGPU_HIP_TRY(hipMemsetAsync(d.conv, 0, sizeof(unsigned long long), s));
+        k_dist_to_bitmap<<<blocks_for(e.nw, tg / kWave, e.grid_cap), tg, 0, s>>>(d, 0, level);
+        k_bitmap_to_queue<<<blocks_for(e.nw, tg, e.grid_cap), tg, 0, s>>>(d, 0, 0);
+        GGPU_HIP_TRY(hipGetLastError());
+        unsigned long long c = 0;
+        if (e.nw) GGPU_HIP_TRY(hipMemcpyAsync(bitmap, d.bm[0], 8ull * e.nw, hipMemcpyDeviceToHost, s));
+        GGPU_HIP_TRY(hipMemcpyAsync(&c, d.conv, sizeof c, hipMemcpyDeviceToHost, s));
+        GGPU_HIP_TRY(hipStreamSynchronize(s));
+        if (c > e.n) throw ArgError("bitmap_to_queue counted more vertices than the graph has", GGPU_ESTATE);
+        if (c) {
+            GGPU_HIP_TRY(hipMemcpyAsync(queue, d.q[0], 4ull * c, hipMemcpyDeviceToHost, s));
+            GGPU_HIP_TRY(hipMemcpyAsync(degrees, d.deg[0], 4ull * c, hipMemcpyDeviceToHost, s));
+            GGPU_HIP_TRY(hipStreamSynchronize(s));
+        }
+        *count = (uint32_t)c;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,35 @@
+/* Test hooks of the HIP backend (no reference counterpart; not part of the
+ * drop-in boundary): the level-loop primitives run on their own, as the
+ * reference's tests run its Primitives (tests/test_main.cpp:355-460).
+ * Callers: tests/ only. */
+#ifndef GGPU_HIP_TESTING_H
+#define GGPU_HIP_TESTING_H
+
+#include "ggpu_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The in-place exclusive scan the edge-balanced top-down runs over frontier
+ * degrees (two passes over tiles of 4096 values: tile sums, then each tile's
+ * carry-in and its own scan), on n values copied in from `in` and out to
+ * `out`; uint32 arithmetic wraps. max_blocks caps the grid (0: the engine's
+ * cap; a small cap makes blocks loop over several tiles). Replaces the
+ * reference test's Primitives::scanExclusive calls (tests/test_main.cpp:382-420). */
+int ggpu_test_scan(ggpu_ctx* ctx, uint32_t n, const uint32_t* in, uint32_t* out, uint32_t max_blocks);
+
+/* The host loop's frontier conversions on the graph of b, with workgroups of
+ * tg threads: dist (n depths) is uploaded, the frontier bitmap of depth `level`
+ * is built (bitmap: ceil(n/64) words; the device words start as all ones, so
+ * every word must be written), then turned into a queue of vertex ids with
+ * their degrees (queue, degrees: room for n entries; *count = its length, in
+ * no particular order). Replaces queueToBitmap / bitmapToQueue in the
+ * reference test (tests/test_main.cpp:422-460). */
+int ggpu_test_frontier_convert(ggpu_bfs* b, const uint32_t* dist, uint32_t level, uint32_t tg, uint64_t* bitmap,
+                               uint32_t* queue, uint32_t* degrees, uint32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

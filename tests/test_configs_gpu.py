@@ -7,22 +7,31 @@
 * grid_c3 (1024 x 2048 grid, p = 0.7, largest component, 2390 levels):
   gpu_do depths and per-level (direction, frontier) through the host,
   persistent and graph loops and hybrid; gpu_pull_scalar for 20 iterations.
-* rmat22_c2 (scale 22, ef 8, permuted): single-source gpu_do, the 64-source
-  batch under the learned policy (six sources against the oracle; all 64 rows
-  equal to the caller-numbering batch and, like all 256 of a 256-source
-  batch, to each source's own sequential gpu_do run), gpu_pull_gs to tol 1e-6
-  judged against the converged FP64 ranks.
-* rmat23_c5 (scale 23, ef 16, a = .60): Jacobi certified to L1 < 1e-4; the
-  64-seed personalized-PageRank block: every column keeps mass 1, two work
-  assignments (given numbering, no chunks / degree numbering, 1024-entry
-  chunks, lane groups) agree column by column, and one column matches the
-  oracle's FP64 PPR after 6 sweeps.
+* rmat22_c2 (scale 22, ef 8, permuted): for every one of the 256 pickSources
+  sources, the row of a 256-source batch against the oracle's serial BFS, and
+  for the first 64 also single-source gpu_do and the row of the 64-source
+  batch (learned policy; the 64 rows also equal the batch traversed in the
+  caller's numbering); gpu_pull_gs to tol 1e-6 judged against the converged
+  FP64 ranks.
+* rmat23_c5 (scale 23, ef 16, a = .60): Jacobi certified to L1 < 1e-4, and
+  in-place Gauss-Seidel (gpu_pull_gs) to L1 < 1e-4 and no worse than Jacobi
+  at the same tolerance; the 64-seed personalized-PageRank block: every
+  column keeps mass 1, two work assignments (given numbering, no chunks /
+  degree numbering, 1024-entry chunks, lane groups) agree column by column,
+  and eight columns match the oracle's FP64 PPR after 6 sweeps.
+
+The oracle runs at these sizes take seconds each (a serial BFS of rmat22:
+about 1 s, a 6-sweep PPR column of rmat23: about 11 s), so they run on a pool
+of host threads (the C calls release the interpreter lock).
 
 Checkers: the oracle's serial BFS and its cpu_do level restatement; the
 FP64 PageRank of the compiled reference (oracle/_ref, OpenMP) when it is
 built, else the oracle's. Acceptance is the reference's checkRanks rule
 (src/cpu/pagerank_cpu.hpp:37-60) and gsNoWorseThanJacobi (:80-84).
 """
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 
@@ -36,6 +45,21 @@ def _pr64(rp, ci, od, iters, tol):
     if O.ref_lib() is not None:
         return O.ref_pagerank(rp, ci, od, 0.85, iters, tol, parallel=True)[0]
     return O.pagerank(rp, ci, od, 0.85, iters, tol)[0]
+
+
+def _workers(cap=16):
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return max(1, min(cap, cores))
+
+
+def _oracle_rows(rp, ci, sources):
+    """(i, source, the oracle's serial BFS depths) for every source, in order,
+    computed on a pool of host threads."""
+    with ThreadPoolExecutor(_workers()) as ex:
+        yield from zip(range(len(sources)), sources, ex.map(lambda s: O.bfs_serial(rp, ci, s), sources))
 
 
 def test_rmat16_c1_gpu_do_64_sources_and_pagerank_20():
@@ -89,35 +113,25 @@ def test_rmat22_c2_gpu_do_batch_and_gs():
     g = G.Graph.named("rmat22_c2")
     rp, ci = g.row_ptr, g.col_idx
     bfs = G.GpuBfs(g)
-    src = g.pick_sources(64, 2)
-    assert len(src) == 64
-    checked = [int(s) for s in src[:6]]
-    refs = {s: O.bfs_serial(rp, ci, s) for s in checked}
-    for s in checked[:3]:
-        r = bfs.run("gpu_do", s)
-        assert np.array_equal(r.dist, refs[s]), s
+    src256 = [int(s) for s in g.pick_sources(256, 2)]
+    src = [int(s) for s in g.pick_sources(64, 2)]
+    assert len(src256) == 256 and src == src256[:64]
     G.install_policies(bfs.ctx)
-    dist, res, _ = bfs.run_batch(src, G.bfs_options(policy=G.POLICY_LEARNED))
-    for i, s in enumerate(checked):
-        assert np.array_equal(dist[i], refs[s]), (i, s)
-    # all 64 rows: the default run (degree numbering, depths un-permuted) equals the
-    # batch traversed in the caller's numbering, which has neither step
-    given, _, _ = bfs.run_batch(src, G.bfs_options(policy=G.POLICY_LEARNED, batch_order=G.BATCH_ORDER_GIVEN))
+    learned = G.bfs_options(policy=G.POLICY_LEARNED)
+    dist, _, _ = bfs.run_batch(np.array(src, dtype=np.uint32), learned)
+    # the default run (degree numbering, depths un-permuted) equals the batch
+    # traversed in the caller's numbering, which has neither step
+    given, _, _ = bfs.run_batch(np.array(src, dtype=np.uint32),
+                                G.bfs_options(policy=G.POLICY_LEARNED, batch_order=G.BATCH_ORDER_GIVEN))
     assert np.array_equal(dist, given)
     del given
-    # every source of the 64 and of a 256-source batch bit-exact against its own
-    # sequential heuristic gpu_do run
-    seq = {}
-    for i, s in enumerate(src):
-        seq[int(s)] = bfs.run("gpu_do", int(s), trace=False).dist
-        assert np.array_equal(dist[i], seq[int(s)]), (i, int(s))
-    del dist
-    src256 = g.pick_sources(256, 2)
-    dist256, _, _ = bfs.run_batch(src256, G.bfs_options(policy=G.POLICY_LEARNED))
-    for i, s in enumerate(src256):
-        want = seq[int(s)] if int(s) in seq else bfs.run("gpu_do", int(s), trace=False).dist
-        assert np.array_equal(dist256[i], want), (i, int(s))
-    del dist256, seq
+    dist256, _, _ = bfs.run_batch(np.array(src256, dtype=np.uint32), learned)
+    for i, s, ref in _oracle_rows(rp, ci, src256):
+        assert np.array_equal(dist256[i], ref), ("256-source batch", i, s)
+        if i < 64:
+            assert np.array_equal(dist[i], ref), ("64-source batch", i, s)
+            assert np.array_equal(bfs.run("gpu_do", s, trace=False).dist, ref), ("gpu_do", s)
+    del dist, dist256
     od = g.degrees()
     conv = _pr64(rp, ci, od, 1000, 1e-10)
     jac = _pr64(rp, ci, od, 1000, 1e-6)
@@ -145,6 +159,17 @@ def test_rmat23_c5_certified_jacobi():
         l1, rel, s = O.check_ranks(conv, pr.ranks())
         assert not res.floor_stop and res.last_delta < tol_cert, (layout, res.last_delta)
         assert l1 < 1e-4 and rel < 1e-3 and abs(s - 1) < 1e-4, (layout, l1, rel, s)
+    # in-place Gauss-Seidel to tol 1e-6: within the reference's acceptance of the
+    # converged ranks, and no worse than FP64 Jacobi stopped at the same tolerance
+    for order in (G.PR_ORDER_DEGREE, G.PR_ORDER_GIVEN):
+        res = pr.run("gpu_pull_gs", G.pr_options(tol=1e-6, iters=1000, order=order))
+        c = O.check_ranks(conv, pr.ranks())
+        assert not res.floor_stop and c[0] < 1e-4 and O.ranks_ok(*c), (order, res.iters, c)
+    jac = _pr64(rp, ci, od, 1000, 1e-6)
+    top = np.argsort(-conv, kind="stable")[:1000]
+    jerr = (float(np.abs(conv - jac).sum()), float((np.abs(conv[top] - jac[top]) / conv[top]).max()),
+            float(jac.sum()))
+    assert O.gs_no_worse(c, jerr), (c, jerr)
 
 
 def test_rmat23_c5_ppr_block():
@@ -158,9 +183,12 @@ def test_rmat23_c5_ppr_block():
     a = pr.ppr_block()
     sums = a.sum(axis=0, dtype=np.float64)
     assert np.abs(sums - 1).max() < 1e-4, np.abs(sums - 1).max()
-    ref, _, _ = O.ppr(g.row_ptr, g.col_idx, od, int(seeds[0]), 0.85, sweeps)
-    err = float(np.abs(a[:, 0].astype(np.float64) - ref).sum())
-    assert err < 1e-5, err
+    cols = list(range(0, B, B // 8))  # eight columns spread over the block
+    with ThreadPoolExecutor(_workers(8)) as ex:
+        refs = ex.map(lambda j: O.ppr(g.row_ptr, g.col_idx, od, int(seeds[j]), 0.85, sweeps)[0], cols)
+        for j, ref in zip(cols, refs):
+            err = float(np.abs(a[:, j].astype(np.float64) - ref).sum())
+            assert err < 1e-5, (j, int(seeds[j]), err)
     pr.run_ppr(seeds, G.pr_options(iters=sweeps, order=G.PR_ORDER_GIVEN, large_min=0xFFFFFFFF, tg=256))
     b = pr.ppr_block()
     diff = np.abs(a.astype(np.float64) - b).sum(axis=0)
