@@ -9,9 +9,13 @@ deduplicated, self-loops removed, symmetrized, ids permuted from seed 2;
 One step = one batch of the 64 sources through BFS on the GPU, inputs
 already in HBM. `value` is the aggregate GTEPS of the batch (sum over sources
 of the component's undirected edges / batch time). Beside it the line carries
-the plain per-source gpu_do rate, PageRank time to L1 < 1e-4 against the FP64
-oracle, the roofline of the dominant kernel (HIP events on the library's
-stream) and the reference's own cpu_do / cpu_omp timed on this host.
+the plain per-source gpu_do rate (and gpu_do under the host and the persistent
+loops, heuristic and learned, with its roofline), the batch in the caller's
+numbering, the level loops on the grid graph, PageRank time to L1 < 1e-4
+against the FP64 oracle on the scale-23 graph and ms per iteration on the
+small RMAT, scale-22 RMAT and grid graphs, the personalized-PageRank block,
+the roofline of the dominant kernel (HIP events on the library's stream) and
+the reference's own cpu_do / cpu_omp timed on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-pagerank]
                        [--only bfs|pagerank|loops|ppr|prw]
@@ -20,7 +24,8 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-pagerank]
        python bench.py --gather-calibration (random 128-byte and 8-byte gathers of known size)
 The roofline's `traffic` (HBM bytes per launch) is read from
 profiles/hbm_traffic.json, which holds the rocprofv3 FETCH_SIZE and WRITE_SIZE
-readings of the dominant kernels scaled by the copy calibration.
+readings of the dominant kernels, FETCH_SIZE scaled by the random-gather
+calibration (the copy calibration, twice as large, is listed beside it).
 """
 import argparse
 import json
@@ -113,6 +118,8 @@ def main():
                     help="random 128-byte and 8-byte gathers from a 1 GiB table (counter calibration runs)")
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 PageRank reference (profiling runs)")
     ap.add_argument("--no-loops", action="store_true", help="skip the level-loop comparison on the grid graph")
+    ap.add_argument("--loop-modes", default="host,graph_k8,graph_k64,persistent,hybrid",
+                    help="level loops the grid comparison runs (comma list)")
     ap.add_argument("--no-pr-workloads", action="store_true",
                     help="skip PageRank ms/iteration on the small RMAT, scale-22 RMAT and grid graphs")
     ap.add_argument("--no-ppr", action="store_true", help="skip the personalized-PageRank block")
@@ -140,7 +147,7 @@ def main():
     if args.only in ("all", "bfs"):
         bfs_res = bfs_part(G, O, args, ctx)
     if args.only == "loops" or (args.only == "all" and not args.no_loops):
-        loops = loops_part(G, O, ctx)
+        loops = loops_part(G, O, ctx, modes=args.loop_modes.split(","))
     if args.only == "pagerank" or (args.only == "all" and not args.no_pagerank):
         pg = G.Graph.named(args.pr_graph)
         pr_res = pagerank_part(G, O, pg, ctx, args.pr_graph, with_ref=not args.no_ref)
@@ -321,14 +328,15 @@ def bfs_part(G, O, args, ctx):
         ms256 = float(np.median([bfs.run_batch(s256, bopts, fetch=False)[1].gpu_ms for _ in range(3)]))
         out["batch256"] = {"sources": len(s256), "gpu_ms": ms256,
                            "gteps": len(s256) * comp_edges / (ms256 / 1e3) / 1e9}
-        # one batch = one traversal launch + one un-permute launch (events around
-        # both): 4(n+1) + 4m + 4*n*S algorithmic bytes (one CSR pass, S depth rows)
+        # one batch = the traversal launch, the depth pass and the un-permute (events
+        # around the three): 4(n+1) + 4m + 4*n*S algorithmic bytes (one CSR pass, S depth rows)
         bytes_batch = 4 * (n + 1) + 4 * m + 4 * n * len(sources)
         avg_ms = float(np.mean(kernel_ms))
         ach = bytes_batch / (avg_ms / 1e3) / 1e9
-        roof = {"kernel": "k_msbfs<1> + k_ms_unpermute<1> (one batch)", "bound": "hbm", "achieved": ach,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": measured_traffic("k_msbfs<1>+k_ms_unpermute<1>", f"{args.graph}/k{len(sources)}"),
+        roof = {"kernel": "k_msbfs<1> + k_ms_finish<1> + k_ms_unpermute<1> (one batch)", "bound": "hbm",
+                "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": measured_traffic("k_msbfs<1>+k_ms_finish<1>+k_ms_unpermute<1>",
+                                            f"{args.graph}/k{len(sources)}"),
                 "bytes_per_launch": bytes_batch, "avg_launch_ms": avg_ms}
     out["roofline"] = roof
     return out
@@ -413,20 +421,26 @@ def pr_workloads_part(G, O, ctx, iters=20, trials=3):
     return out
 
 
-def loops_part(G, O, ctx, trials=3):
+def loops_part(G, O, ctx, trials=3, modes=None):
     """gpu_do on the grid graph (>1500 levels) under each level loop: ms per traversal and us per level."""
     g = G.Graph.named("grid_c3")
     bfs = G.GpuBfs(g, ctx)
     src = [int(s) for s in g.pick_sources(4, 3)]
     out = {"workload": f"grid_c3: largest component of a 1024 x 2048 grid, edges kept with p = 0.7 (seed 3); "
                        f"n {g.n}, m {g.m}; gpu_do from {len(src)} pickSources sources (seed 3)"}
-    modes = [("host", dict(loop=G.LOOP_HOST, time_levels=0), "gpu_do"),
-             ("graph_k8", dict(loop=G.LOOP_GRAPH, graph_levels=8), "gpu_do"),
-             ("graph_k64", dict(loop=G.LOOP_GRAPH, graph_levels=64), "gpu_do"),
-             ("persistent", dict(loop=G.LOOP_PERSISTENT), "gpu_do"),
-             ("hybrid", dict(loop=G.LOOP_HOST, time_levels=0), "hybrid")]
+    all_modes = [("host", dict(loop=G.LOOP_HOST, time_levels=0), "gpu_do"),
+                 ("graph_k8", dict(loop=G.LOOP_GRAPH, graph_levels=8), "gpu_do"),
+                 ("graph_k64", dict(loop=G.LOOP_GRAPH, graph_levels=64), "gpu_do"),
+                 ("persistent", dict(loop=G.LOOP_PERSISTENT), "gpu_do"),
+                 ("hybrid", dict(loop=G.LOOP_HOST, time_levels=0), "hybrid")]
+    if modes:
+        unknown = set(modes) - {k for k, _, _ in all_modes}
+        if unknown:
+            raise SystemExit(f"unknown loop modes {sorted(unknown)}")
     ref = O.bfs_serial(g.row_ptr, g.col_idx, src[0])
-    for key, kw, variant in modes:
+    for key, kw, variant in all_modes:
+        if modes and key not in modes:
+            continue
         o = G.bfs_options(**kw)
         r = bfs.run(variant, src[0], o, trace=False)
         exact = bool(np.array_equal(r.dist, ref))
@@ -460,7 +474,7 @@ def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
     bytes_sweep = 4 * (n + 1) + 8 * m + 4 * n
     # per-sweep time of the fused Jacobi sweep per layout and vertex order (20 sweeps, no host sync)
     sweeps, combos = {}, {}
-    for layout, lname in ((G.PR_LAYOUT_STREAM, "stream"), (G.PR_LAYOUT_HOT, "hot")):
+    for layout, lname in ((G.PR_LAYOUT_STREAM, "stream"), (G.PR_LAYOUT_HOT, "hot"), (G.PR_LAYOUT_TILED, "tiled")):
         for order, oname in ((G.PR_ORDER_GIVEN, "given"), (G.PR_ORDER_DEGREE, "degree")):
             name = f"{lname}/{oname}"
             fo = G.pr_options(iters=20, order=order, layout=layout)
@@ -539,7 +553,7 @@ def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
         log(f"  cpu_omp ({cores} cores): {per_it:.1f} ms/iteration, L1 < 1e-4 after {k} iterations in {ms:.1f} ms")
     per_sweep_ms = sweeps[best_name]
     ach = bytes_sweep / (per_sweep_ms / 1e3) / 1e9
-    kname = "k_pr_sweep_hot" if layout == G.PR_LAYOUT_HOT else "k_pr_sweep"
+    kname = {G.PR_LAYOUT_HOT: "k_pr_sweep_hot", G.PR_LAYOUT_TILED: "k_pr_sweep_tiled"}.get(layout, "k_pr_sweep")
     res["roofline"] = {"kernel": f"{kname} (gpu_pull_adaptive, Jacobi, {best_name} layout/order)",
                        "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(kname, g_name),
