@@ -12,8 +12,9 @@
 //  * cpu_omp / cpu_do discover a vertex with the fetch_or on the visited
 //    bitmap that sets its bit (exactly once; dist gets a plain store), and a
 //    level sets the next frontier's bits; no per-thread queues;
-//  * every queue is collected from a bitmap in increasing vertex order, so a
-//    level's output does not depend on thread timing;
+//  * a queue made by several threads is collected from a bitmap in
+//    increasing vertex order, and a one-thread top-down step appends in
+//    discovery order, so no level's output depends on thread timing;
 //  * work is dealt in units (words of 64 vertices, or queue entries) by one
 //    atomic cursor: a static split, chunks of 64 units, or guided halving
 //    (--schedule static|dynamic|guided);
@@ -90,6 +91,7 @@ inline bool test_bit(const uint64_t* w, uint32_t v) { return (w[v >> 6] >> (v & 
 struct Frontier {
     uint64_t n = 0, nw = 0;
     std::vector<uint64_t> w, sum;
+    mutable std::vector<uint64_t> at;    // collect's prefix of summary words
     explicit Frontier(uint64_t n_ = 0) { resize(n_); }
     void resize(uint64_t n_) {
         n = n_;
@@ -117,6 +119,8 @@ struct Frontier {
         w[i] = bits;
         if (bits) __atomic_fetch_or(&sum[i >> 6], 1ull << (i & 63), __ATOMIC_RELAXED);
     }
+    // words that are not zero (counted when needed: a shared counter updated by
+    // every insert costs the parallel levels a contended atomic per new word)
     uint64_t words_set() const {
         uint64_t c = 0;
         for (uint64_t x : sum) c += (uint64_t)__builtin_popcountll(x);
@@ -136,7 +140,21 @@ struct Frontier {
     uint64_t collect(const Csr& g, uint32_t* q, uint32_t* deg, int nt) const {
         nt = team_for(nt);
         const uint64_t ns = sum.size();
-        std::vector<uint64_t> at(ns + 1, 0);
+        if (nt == 1) {  // one pass
+            uint64_t k = 0;
+            for (uint64_t s = 0; s < ns; ++s)
+                for (uint64_t bits = sum[s]; bits; bits &= bits - 1) {
+                    const uint64_t i = s * 64 + (uint64_t)__builtin_ctzll(bits);
+                    for (uint64_t m = w[i]; m; m &= m - 1) {
+                        const uint32_t v = (uint32_t)(i * 64 + (uint64_t)__builtin_ctzll(m));
+                        q[k] = v;
+                        if (deg) deg[k] = g.deg(v);
+                        ++k;
+                    }
+                }
+            return k;
+        }
+        at.assign(ns + 1, 0);
         deal(nt, Schedule::Dynamic64, ns, [&](uint64_t b, uint64_t e, Counts&) {
             for (uint64_t s = b; s < e; ++s) {
                 uint64_t c = 0;
@@ -349,13 +367,39 @@ void init_dist(uint32_t* dist, uint64_t n, uint32_t src, int threads) {
     dist[src] = 0;
 }
 
-// The hybrid mirror keeps no visited bitmap: discovery is the CAS of dist from
-// INF; discoveries go to the thread's scratch set, collected in vertex order.
+// The hybrid mirror keeps no visited bitmap: discovery is the first write of
+// dist from INF. A small level (few edges) runs on one thread and appends each
+// discovery to the next queue as it is made (that order depends only on the
+// input queue); a large one claims with a CAS, puts discoveries in the thread's
+// scratch set and collects them in vertex order.
 Out top_down(const Csr& g, uint32_t* dist, const uint32_t* q, uint64_t size, uint64_t edges, uint32_t* q_next,
              uint32_t* deg_next, uint32_t level, int threads) {
     const int nt = edges < kSerialEdges ? 1 : team(threads);
-    Frontier& S = scratch_set(g.n);
     const uint32_t depth = level + 1;
+    Out out;
+    if (nt == 1) {
+        uint64_t found = 0;
+        const Counts c = deal(1, Schedule::Static, size, [&](uint64_t b, uint64_t e, Counts& k) {
+            for (uint64_t i = b; i < e; ++i) {
+                const uint32_t v = q[i];
+                k.examined += g.deg(v);
+                for (const uint32_t* p = g.nbr_begin(v); p != g.nbr_end(v); ++p) {
+                    if (dist[*p] != kInf) continue;
+                    dist[*p] = depth;
+                    const uint32_t dg = g.deg(*p);
+                    q_next[found] = *p;
+                    deg_next[found] = dg;
+                    ++found;
+                    k.next_edges += dg;
+                }
+            }
+        });
+        out.found = found;
+        out.examined = c.examined;
+        out.next_edges = c.next_edges;
+        return out;
+    }
+    Frontier& S = scratch_set(g.n);
     const Counts c = deal(nt, Schedule::Dynamic64, size, [&](uint64_t b, uint64_t e, Counts& k) {
         for (uint64_t i = b; i < e; ++i) {
             const uint32_t v = q[i];
@@ -370,7 +414,6 @@ Out top_down(const Csr& g, uint32_t* dist, const uint32_t* q, uint64_t size, uin
             }
         }
     });
-    Out out;
     out.found = S.collect(g, q_next, deg_next, nt);
     out.examined = c.examined;
     out.next_edges = c.next_edges;

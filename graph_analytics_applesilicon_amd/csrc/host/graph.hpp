@@ -130,7 +130,7 @@ void bfs_do(const Csr& g, uint32_t src, uint32_t* dist, const CpuBfsOpts& o, std
 // word v/64) for bottom-up.
 namespace steps {
 struct Out { uint64_t found = 0, examined = 0, next_edges = 0; };
-constexpr uint64_t kSerialEdges = 2048;
+constexpr uint64_t kSerialEdges = 8192;
 void init_dist(uint32_t* dist, uint64_t n, uint32_t src, int threads);
 Out top_down(const Csr& g, uint32_t* dist, const uint32_t* q, uint64_t size, uint64_t edges, uint32_t* q_next,
              uint32_t* deg_next, uint32_t level, int threads);
