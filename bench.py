@@ -474,17 +474,21 @@ def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
     bytes_sweep = 4 * (n + 1) + 8 * m + 4 * n
     # per-sweep time of the fused Jacobi sweep per layout and vertex order (20 sweeps, no host sync)
     sweeps, combos = {}, {}
-    for layout, lname in ((G.PR_LAYOUT_STREAM, "stream"), (G.PR_LAYOUT_HOT, "hot"), (G.PR_LAYOUT_TILED, "tiled")):
+    for layout, lname, lmin in ((G.PR_LAYOUT_STREAM, "stream", 256), (G.PR_LAYOUT_HOT, "hot", 256),
+                                (G.PR_LAYOUT_TILED, "tiled", 256), (G.PR_LAYOUT_HOT_TAIL, "hot_tail64", 64),
+                                (G.PR_LAYOUT_HOT_TAIL, "hot_tail256", 256)):
         for order, oname in ((G.PR_ORDER_GIVEN, "given"), (G.PR_ORDER_DEGREE, "degree")):
+            if layout == G.PR_LAYOUT_HOT_TAIL and order == G.PR_ORDER_GIVEN:
+                continue  # built for the degree order (hot sources and hub rows first)
             name = f"{lname}/{oname}"
-            fo = G.pr_options(iters=20, order=order, layout=layout)
+            fo = G.pr_options(iters=20, order=order, layout=layout, large_min=lmin)
             pr.run("gpu_pull_adaptive", fo)
             r = pr.run("gpu_pull_adaptive", fo)
             sweeps[name] = r.gpu_ms / 20
-            combos[name] = (layout, order)
+            combos[name] = (layout, order, lmin)
             log(f"  sweep ({name}): {sweeps[name] * 1e3:.1f} us")
     best_name = min(sweeps, key=sweeps.get)
-    layout, order = combos[best_name]
+    layout, order, lmin = combos[best_name]
     res["sweep_ms"] = sweeps
     # gpu_push (FP64 atomics along the out-arcs) on the degree-ordered copy, for comparison
     po = G.pr_options(iters=20, order=G.PR_ORDER_DEGREE)
@@ -494,7 +498,7 @@ def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
     # Jacobi, certified stop: ||r_k - r*||_1 <= d/(1-d) * delta_k  =>  delta_k < 1e-4 * (1-d)/d
     tol_cert = 1e-4 * 0.15 / 0.85
     for variant, opts, key in (("gpu_pull_adaptive", G.pr_options(tol=tol_cert, iters=1000, order=order,
-                                                                  layout=layout), "jacobi_certified"),
+                                                                  layout=layout, large_min=lmin), "jacobi_certified"),
                                ("gpu_pull_gs", G.pr_options(tol=1e-6, iters=1000, order=order), "gs_tol_1e-6")):
         pr.run(variant, opts)  # warm-up (builds the order)
         t1 = time.perf_counter()
@@ -511,7 +515,7 @@ def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
     # sweeps). The certified runs above need no oracle and stop later.
     if ref is not None:
         for variant, key, mk in (("gpu_pull_adaptive", "jacobi_kstar",
-                                  lambda k: G.pr_options(iters=k, order=order, layout=layout)),
+                                  lambda k: G.pr_options(iters=k, order=order, layout=layout, large_min=lmin)),
                                  ("gpu_pull_gs", "gs_kstar", lambda k: G.pr_options(iters=k, order=order))):
             kstar = None
             for k in range(1, 61):
@@ -553,7 +557,8 @@ def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
         log(f"  cpu_omp ({cores} cores): {per_it:.1f} ms/iteration, L1 < 1e-4 after {k} iterations in {ms:.1f} ms")
     per_sweep_ms = sweeps[best_name]
     ach = bytes_sweep / (per_sweep_ms / 1e3) / 1e9
-    kname = {G.PR_LAYOUT_HOT: "k_pr_sweep_hot", G.PR_LAYOUT_TILED: "k_pr_sweep_tiled"}.get(layout, "k_pr_sweep")
+    kname = {G.PR_LAYOUT_HOT: "k_pr_sweep_hot", G.PR_LAYOUT_TILED: "k_pr_sweep_tiled",
+             G.PR_LAYOUT_HOT_TAIL: "k_pr_sweep_hot + k_pr_tail + k_pr_tail_merge"}.get(layout, "k_pr_sweep")
     res["roofline"] = {"kernel": f"{kname} (gpu_pull_adaptive, Jacobi, {best_name} layout/order)",
                        "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(kname, g_name),

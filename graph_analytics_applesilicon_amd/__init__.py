@@ -41,7 +41,7 @@ POLICY_HEURISTIC, POLICY_LEARNED, POLICY_SCRIPTED = 0, 1, 2
 BATCH_ORDER_GIVEN, BATCH_ORDER_DEGREE, BATCH_ORDER_AUTO = 0, 1, 2
 ACTION_TD_EDGE, ACTION_TD_BINNED, ACTION_BU = 0, 1, 2
 PR_SCALAR, PR_VECTOR, PR_ADAPTIVE, PR_GS, PR_PUSH = 1, 2, 3, 4, 5
-PR_LAYOUT_STREAM, PR_LAYOUT_BINS, PR_LAYOUT_HOT, PR_LAYOUT_TILED = 0, 1, 2, 3
+PR_LAYOUT_STREAM, PR_LAYOUT_BINS, PR_LAYOUT_HOT, PR_LAYOUT_TILED, PR_LAYOUT_HOT_TAIL = 0, 1, 2, 3, 4
 PR_ORDER_GIVEN, PR_ORDER_DEGREE = 0, 1
 PR_VARIANTS = {"gpu_pull_scalar": PR_SCALAR, "gpu_pull_vector": PR_VECTOR,
                "gpu_pull_adaptive": PR_ADAPTIVE, "gpu_pull_gs": PR_GS, "gpu_push": PR_PUSH}
@@ -194,6 +194,8 @@ def _optional_hip_symbols():
         "ggpu_test_scan": ([vp, C.c_uint32, _u32p, _u32p, C.c_uint32], C.c_int),
         "ggpu_test_frontier_convert": ([vp, _u32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64), _u32p, _u32p,
                                         C.POINTER(C.c_uint32)], C.c_int),
+        "ggpu_test_pr_tail_geometry": ([vp, C.c_uint32, C.c_uint32], C.c_int),
+        "ggpu_test_pr_tail_info": ([vp, C.c_uint32, C.POINTER(C.c_uint64)], C.c_int),
         "ggpu_bfs_run_noalloc": ([vp, C.c_uint64, C.c_uint64, vp, vp, C.c_uint32, C.POINTER(BfsOpts), vp,
                                   C.POINTER(C.c_uint32)], C.c_int),
         "ggpu_pr_opts_default": ([C.POINTER(PrOpts)], None),

@@ -23,9 +23,12 @@
 #include <memory>
 #include <numeric>
 
+#include "ggpu_hip_testing.h"
 #include "graph.hpp"
 #include "pagerank_kernels.hpp"
 #include "pr_shared.hpp"
+#include "pr_tail.hpp"
+#include "pr_tail_plan.hpp"
 #include "pr_tiled.hpp"
 #include "pr_tiling.hpp"
 
@@ -35,6 +38,12 @@ constexpr uint32_t kSecBlocks = 2048;             // block cap per section (grid
 constexpr uint32_t kStreamBlocks = 8192;          // block cap of a stream-layout sweep
 constexpr uint32_t kPartialCap = 1u << 17;        // FP64 partial slots per quantity (all launches of a sweep)
 constexpr uint32_t kPushHeavy = 256;              // gpu_push: out-degree from which a wave pushes a source
+// HOT_TAIL geometry: sources below kTailBegin are the LDS copy (30720) and a
+// ~2 MB band that stays in L2; tail tiles of kTailTile sources are 2 MB of
+// contributions each (one XCD's L2 is 4 MB).
+constexpr uint32_t kTailBegin = 1u << 19;
+constexpr uint32_t kTailTile = 1u << 19;
+constexpr uint32_t kTailMergeBlocks = 16384;       // block cap of the tail merge (grid-stride beyond)
 
 // ------------------------------------------------------------------ kernels
 __global__ void k_pr_sweep(PrSweep P) { pr_sweep_body(P); }
@@ -43,6 +52,8 @@ __global__ void __launch_bounds__(kHotThreads) k_pr_sweep_hot(PrSweep P) { pr_ho
 __global__ void k_pr_longfix(PrSweep P) { pr_longfix_body(P); }
 __global__ void __launch_bounds__(kTileThreads) k_pr_sweep_tiled(PrSweep P, PrTiledDev Z) { pr_tiled_body(P, Z); }
 __global__ void __launch_bounds__(256) k_pr_tiled_merge(PrSweep P, PrTiledDev Z) { pr_tiled_merge_body(P, Z); }
+__global__ void __launch_bounds__(kTailWaves * kWave) k_pr_tail(PrSweep P, PrTailDev Z) { pr_tail_body(P, Z); }
+__global__ void __launch_bounds__(256) k_pr_tail_merge(PrSweep P, PrTailDev Z) { pr_tail_merge_body(P, Z); }
 __global__ void k_pr_reduce(const double* partial, uint32_t cap, uint32_t count, PrAcc* out) {
     pr_reduce_body(partial, cap, count, out);
 }
@@ -171,6 +182,36 @@ struct TiledLayout {
     }
 };
 
+// The HOT_TAIL layout of one numbering: the HOT items with the split rows'
+// heads as flagged long rows (stream), the tail plan (pr_tail_plan.hpp) on the
+// device, and the partial array.
+struct TailLayout {
+    uint32_t min_indeg = 0, tail_begin = 0, tile = 0;
+    StreamLayout stream;
+    uint32_t R = 0, T = 0, n_items = 0, n_big = 0;
+    uint64_t tail_arcs = 0;
+    DBuf<uint32_t> tci, px, pdst, srow, slr, ovptr, big;
+    DBuf<uint2> items;
+    DBuf<float2> part;
+    PrTailDev dev() const {
+        PrTailDev Z{};
+        Z.tci = tci.p;
+        Z.px = px.p;
+        Z.pdst = pdst.p;
+        Z.items = items.p;
+        Z.n_items = n_items;
+        Z.part = part.p;
+        Z.srow = srow.p;
+        Z.slr = slr.p;
+        Z.ovptr = ovptr.p;
+        Z.big = big.p;
+        Z.n_big = n_big;
+        Z.R = R;
+        Z.T = T;
+        return Z;
+    }
+};
+
 template <class T, class H>
 static void upload_vec(DBuf<T>& d, const std::vector<H>& h) {
     static_assert(sizeof(T) == sizeof(H), "host and device records differ in size");
@@ -195,6 +236,7 @@ struct Space {
     std::vector<std::unique_ptr<StreamLayout>> stream;  // one per range
     std::unique_ptr<StreamLayout> wstream;              // HOT layout: wave-sized items over all rows
     std::unique_ptr<TiledLayout> tiled;                 // TILED layout
+    std::unique_ptr<TailLayout> tail;                   // HOT_TAIL layout
     DBuf<uint32_t> bins;
     uint32_t bin_count[3] = {0, 0, 0};
     // gpu_push: out-arcs (the transpose of `in`), sources of out-degree >= kPushHeavy
@@ -215,6 +257,8 @@ struct PrEngine {
     DBuf<double> partial;
     uint32_t npart = 0;      // partial slots written since the last reduction
     DBuf<PrAcc> acc;         // [0],[1] Jacobi ping-pong / in-place state, [2] in-place sweep totals
+    // HOT_TAIL geometry: first tail source and sources per tile (test hook overrides; 0: the defaults)
+    uint32_t tail_begin_set = 0, tail_tile_set = 0;
     // Gauss-Seidel decision (made on the caller's numbering)
     bool gs_ready = false, gs_colored = false;
     uint32_t gs_pair_k = 0;
@@ -279,6 +323,7 @@ struct PrEngine {
         S.stream.clear();
         S.wstream.reset();
         S.tiled.reset();
+        S.tail.reset();
         S.bins_ready = false;
         S.out_ready = false;
     }
@@ -434,6 +479,103 @@ struct PrEngine {
         GGPU_HIP_TRY(hipMemset(T->part.p, 0, std::max<size_t>(np, 1) * sizeof(float2)));
         GGPU_HIP_TRY(hipDeviceSynchronize());
         S.tiled = std::move(T);
+    }
+
+    // The HOT_TAIL layout of space S for split rows of in-degree >= min_indeg:
+    // the HOT items (make_stream's wave-sized blocks), except that a split row's
+    // head [rp[v], split) becomes long-row chunks flagged kLrowSplit (their
+    // partials are always stored), and the tail plan with its partial array.
+    void build_tail(Space& S, uint32_t min_indeg) {
+        const uint32_t tb = tail_begin_set ? tail_begin_set : kTailBegin;
+        const uint32_t tile = tail_tile_set ? tail_tile_set : kTailTile;
+        if (S.tail && S.tail->min_indeg == min_indeg && S.tail->tail_begin == tb && S.tail->tile == tile) return;
+        TailParams prm;
+        prm.min_indeg = min_indeg;
+        prm.tail_begin = tb;
+        prm.tile = tile;
+        prm.nnz_cap = kWaveNnz;
+        prm.pieces_cap = kWaveNnz;
+        const TailPlan plan = plan_tail(S.in, prm);
+        auto T = std::make_unique<TailLayout>();
+        T->min_indeg = min_indeg;
+        T->tail_begin = tb;
+        T->tile = tile;
+        T->R = plan.R;
+        T->T = plan.T;
+        T->n_items = (uint32_t)plan.items.size();
+        T->tail_arcs = plan.tci.size();
+        std::vector<uint2> sitems, lrows;
+        std::vector<uint4> litems;
+        std::vector<uint32_t> slr(plan.R, kNoHead);
+        uint32_t cur0 = 0, rows = 0, nnz = 0, j = 0;
+        bool multi = false;
+        auto close = [&](uint32_t end) {
+            if (rows) sitems.push_back(make_uint2(cur0, end));
+            rows = 0;
+            nnz = 0;
+        };
+        auto chunks = [&](uint32_t v, uint32_t b, uint32_t e, uint32_t flag) {
+            const uint32_t first = (uint32_t)litems.size();
+            for (uint32_t x = b; x < e; x += kWaveNnz)
+                litems.push_back(make_uint4(v, x, std::min(x + kWaveNnz, e), (uint32_t)lrows.size()));
+            const uint32_t cnt = (uint32_t)litems.size() - first;
+            lrows.push_back(make_uint2(first, cnt | flag));
+            return cnt;
+        };
+        for (uint32_t v = 0; v < n; ++v) {
+            const uint32_t b = S.in.rp[v], dg = S.in.deg(v);
+            if (j < plan.R && plan.srow[j] == v) {
+                close(v);
+                if (plan.split[j] > b) {
+                    slr[j] = (uint32_t)lrows.size();
+                    chunks(v, b, plan.split[j], kLrowSplit);
+                }
+                ++j;
+                continue;
+            }
+            if (dg > kWaveNnz) {
+                close(v);
+                multi |= chunks(v, b, b + dg, 0u) > 1;
+                continue;
+            }
+            if (rows && (nnz + dg > kWaveNnz || rows == kWaveNnz)) close(v);
+            if (!rows) cur0 = v;
+            ++rows;
+            nnz += dg;
+        }
+        close(n);
+        StreamLayout& L = T->stream;
+        L.r_begin = 0;
+        L.r_end = n;
+        L.ns = (uint32_t)sitems.size();
+        L.nl = (uint32_t)litems.size();
+        L.nlr = (uint32_t)lrows.size();
+        L.multi = multi;
+        upload_vec(L.sitems, sitems);
+        upload_vec(L.litems, litems);
+        upload_vec(L.lrows, lrows);
+        L.lpart.alloc(L.nl + 1);
+        // split rows with many head or overflow partials are merged by a wave
+        std::vector<uint32_t> big;
+        for (uint32_t r = 0; r < plan.R; ++r) {
+            const uint32_t heads = slr[r] == kNoHead ? 0u : lrows[slr[r]].y & ~kLrowSplit;
+            if (heads + (plan.ovptr[r + 1] - plan.ovptr[r]) > kMergeLanePartials) big.push_back(r);
+        }
+        T->n_big = (uint32_t)big.size();
+        upload_vec(T->tci, plan.tci);
+        upload_vec(T->px, plan.px);
+        upload_vec(T->pdst, plan.pdst);
+        upload_vec(T->items, plan.items);
+        upload_vec(T->srow, plan.srow);
+        upload_vec(T->slr, slr);
+        upload_vec(T->ovptr, plan.ovptr);
+        upload_vec(T->big, big);
+        // tile partials of rows without arcs in a tile stay zero: cleared once here
+        const size_t np = (size_t)plan.T * plan.R + plan.ovptr.back();
+        T->part.alloc(std::max<size_t>(np, 1));
+        GGPU_HIP_TRY(hipMemset(T->part.p, 0, std::max<size_t>(np, 1) * sizeof(float2)));
+        GGPU_HIP_TRY(hipDeviceSynchronize());
+        S.tail = std::move(T);
     }
 
     void build_bins(Space& S, uint32_t small_max, uint32_t large_min) {
@@ -629,6 +771,31 @@ struct PrEngine {
             reduce(acc.p + nx);
             return;
         }
+        if (variant == GGPU_PR_ADAPTIVE && o.layout == GGPU_PR_LAYOUT_HOT_TAIL) {
+            // the HOT sweep (split rows' heads as partials), the tail tiles, then the merge of the split rows
+            build_tail(S, o.large_min);
+            TailLayout& T = *S.tail;
+            launch(P, use_stream(P, T.stream, 2), o.tg);
+            if (T.R) {
+                const PrTailDev Z = T.dev();
+                if (T.n_items) {
+                    const uint32_t g = (T.n_items + kTailWaves - 1) / kTailWaves;  // one item per wave
+                    k_pr_tail<<<g, kTailWaves * kWave, 0, ctx->stream>>>(P, Z);
+                    GGPU_HIP_TRY(hipGetLastError());
+                }
+                // a lane per split row, and a wave per row of the `big` list (no wave
+                // merges several hub rows one after another)
+                const uint64_t want = std::max<uint64_t>((T.R + 255) / 256, ((uint64_t)T.n_big + 3) / 4);
+                const uint32_t gm = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, kTailMergeBlocks));
+                if (npart + gm > kPartialCap) throw ArgError("too many blocks in one sweep");
+                P.partial_base = npart;
+                npart += gm;
+                k_pr_tail_merge<<<gm, 256, 0, ctx->stream>>>(P, Z);
+                GGPU_HIP_TRY(hipGetLastError());
+            }
+            reduce(acc.p + nx);
+            return;
+        }
         const uint32_t grid = jacobi_layout(P, S, variant, o);
         launch(P, grid, o.tg);
         reduce(acc.p + nx);
@@ -682,7 +849,7 @@ struct PrEngine {
         const uint32_t L = o.medium_lanes;
         if (L == 0 || L > 64 || (L & (L - 1))) throw ArgError("medium_lanes must be a power of two in [1, 64]");
         if (o.iters < 0) throw ArgError("iters must be >= 0");
-        if (o.layout > GGPU_PR_LAYOUT_TILED) throw ArgError("unknown PageRank layout");
+        if (o.layout > GGPU_PR_LAYOUT_HOT_TAIL) throw ArgError("unknown PageRank layout");
         if (o.layout == GGPU_PR_LAYOUT_TILED && (o.large_min == 0 || o.large_min > kTileNnz))
             throw ArgError("TILED needs 1 <= large_min <= " + std::to_string(kTileNnz) + " (the in-degree of a tiled row)");
         if (o.order > GGPU_PR_ORDER_DEGREE) throw ArgError("unknown PageRank vertex order");
@@ -790,7 +957,9 @@ struct PrEngine {
                                    {GGPU_PR_LAYOUT_HOT, GGPU_PR_ORDER_DEGREE, 0, 0, 64},
                                    {GGPU_PR_LAYOUT_TILED, GGPU_PR_ORDER_DEGREE, 0, 64, 64},
                                    {GGPU_PR_LAYOUT_TILED, GGPU_PR_ORDER_DEGREE, 0, 128, 64},
-                                   {GGPU_PR_LAYOUT_TILED, GGPU_PR_ORDER_DEGREE, 0, 256, 64}};
+                                   {GGPU_PR_LAYOUT_TILED, GGPU_PR_ORDER_DEGREE, 0, 256, 64},
+                                   {GGPU_PR_LAYOUT_HOT_TAIL, GGPU_PR_ORDER_DEGREE, 0, 64, 64},
+                                   {GGPU_PR_LAYOUT_HOT_TAIL, GGPU_PR_ORDER_DEGREE, 0, 256, 64}};
         for (uint32_t order : {(uint32_t)GGPU_PR_ORDER_GIVEN, (uint32_t)GGPU_PR_ORDER_DEGREE})
             for (uint32_t sm : {4u, 16u})
                 for (uint32_t lg : {256u, 4096u})
@@ -805,7 +974,7 @@ struct PrEngine {
                 t.small_max = c.sm;
                 t.large_min = c.lg;
                 t.medium_lanes = c.L;
-            } else if (c.layout == GGPU_PR_LAYOUT_TILED) {
+            } else if (c.layout == GGPU_PR_LAYOUT_TILED || c.layout == GGPU_PR_LAYOUT_HOT_TAIL) {
                 t.large_min = c.lg;
             }
             t.iters = iters;
@@ -962,6 +1131,28 @@ int ggpu_ppr_autotune(ggpu_pr* p, const uint32_t* seeds, uint32_t B, int iters, 
 int ggpu_ppr_fetch(ggpu_pr* p, float* out) {
     if (!p || !out) return GGPU_EINVAL;
     return guarded(p->ctx, [&] { ppr_fetch(*p->eng, out); });
+}
+
+int ggpu_test_pr_tail_geometry(ggpu_pr* p, uint32_t tail_begin, uint32_t tile) {
+    if (!p) return GGPU_EINVAL;
+    p->eng->tail_begin_set = tail_begin;
+    p->eng->tail_tile_set = tile;
+    return GGPU_OK;
+}
+
+int ggpu_test_pr_tail_info(const ggpu_pr* p, uint32_t order, uint64_t* stats) {
+    if (!p || !stats || order > GGPU_PR_ORDER_DEGREE) return GGPU_EINVAL;
+    const Space& S = p->eng->sp[order == GGPU_PR_ORDER_DEGREE ? SP_DEGREE : SP_GIVEN];
+    for (int i = 0; i < 6; ++i) stats[i] = 0;
+    if (!S.tail) return GGPU_OK;
+    const TailLayout& T = *S.tail;
+    stats[0] = T.R;
+    stats[1] = T.T;
+    stats[2] = T.tail_arcs;
+    stats[3] = T.n_items;
+    stats[4] = T.n_big;
+    stats[5] = T.stream.nl;
+    return GGPU_OK;
 }
 
 }  // extern "C"

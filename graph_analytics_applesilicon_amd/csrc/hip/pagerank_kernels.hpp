@@ -34,12 +34,17 @@ struct PrSection {
     uint32_t blk_end;
 };
 
+// A long row whose item count carries this bit is a HOT_TAIL split row's head:
+// its chunk partials are always stored, and the tail merge (pr_tail.hpp), not
+// pr_longfix_body, combines them with the row's tail partials.
+constexpr uint32_t kLrowSplit = 0x80000000u;
+
 struct PrStreamLayout {
     const uint2* sitems;  // {first row, end row}
     uint32_t n_sitems;
     const uint4* litems;  // {row, first edge, end edge, long-row index}
     uint32_t n_litems;
-    const uint2* lrows;   // {first item, item count} per long row
+    const uint2* lrows;   // {first item, item count (| kLrowSplit)} per long row
     float2* lpart;        // per long item (hi, lo)
     uint32_t n_lrows;
 };
@@ -461,7 +466,8 @@ __device__ __forceinline__ void pr_longfix_body(const PrSweep& P) {
     PrPart part;
     for (uint32_t row = blockIdx.x * wpb + wid; row < Lyt.n_lrows; row += gridDim.x * wpb) {
         const uint2 lr = Lyt.lrows[row];
-        if (lr.y < 2) continue;  // single-chunk rows were stored by the sweep (wave-uniform)
+        // single-chunk rows were stored by the sweep; split rows are the tail merge's (wave-uniform)
+        if (lr.y < 2 || (lr.y & kLrowSplit)) continue;
         float H = 0.f, L = 0.f;
         for (uint32_t j = lane; j < lr.y; j += kWave) {
             const float2 q = Lyt.lpart[lr.x + j];

@@ -1,0 +1,149 @@
+// The HOT_TAIL PageRank layout's device side (plan: csrc/host/pr_tail_plan.hpp).
+// No reference counterpart: the reference pulls every row from the whole
+// contribution vector (src/shaders/pagerank.metal:306-381).
+//
+// In the degree-ordered numbering the tail sources [tail_begin, n) hold a few
+// per cent of the gathers but, spread over most of the contribution vector,
+// every one of them misses L2. The split rows' tail arcs are copied tile-major
+// (tile t: sources [tail_begin + t * tile, + tile), a tile of contributions
+// small enough for one XCD's L2). k_pr_tail runs one wave per item in item
+// order (tile-major), so the waves in flight read one or two tiles, which stay
+// in L2: each wave stages its item's gathers in LDS and sums each piece (<= 512
+// arcs of one split row in one tile) with Kahan + TwoSum into a (hi, lo)
+// partial. The HOT sweep leaves the split rows' heads (sources below
+// tail_begin) as chunk partials; k_pr_tail_merge adds, per split row, its head
+// partials in chunk order, then its tile partials in tile order, then its
+// overflow partials, with TwoSum, and stores the row (pr_store: ranks,
+// contributions and the FP64 dangling / sum / change partials).
+#pragma once
+
+#include "pagerank_kernels.hpp"
+
+namespace ggpu {
+
+constexpr uint32_t kTailWaves = 4;             // waves per k_pr_tail workgroup (one item each)
+constexpr uint32_t kNoHead = 0xFFFFFFFFu;      // a split row whose in-list lies entirely in the tail
+constexpr uint32_t kMergeLanePartials = 8;     // split rows with more head + overflow partials: a wave each
+
+struct PrTailDev {
+    const uint32_t* tci;    // tail arcs (source ids), tile-major
+    const uint32_t* px;     // piece p: tci[px[p], px[p + 1])
+    const uint32_t* pdst;   // piece p: partial index
+    const uint2* items;     // {first piece, end piece}
+    uint32_t n_items;
+    float2* part;           // [T][R] tile partials (zero where a row has no arc in a tile), then overflow slots
+    const uint32_t* srow;   // split row -> vertex
+    const uint32_t* slr;    // split row -> long-row index of its head chunks in the stream layout, or kNoHead
+    const uint32_t* ovptr;  // split row -> overflow slots [ovptr[j], ovptr[j + 1])
+    const uint32_t* big;    // split rows merged by a whole wave
+    uint32_t n_big;
+    uint32_t R, T;
+};
+
+// One wave per item: the item's arcs are gathered (kGatherBatch loads in flight
+// per lane, tci streamed with non-temporal loads) into the wave's LDS staging,
+// then G lanes per piece sum it (G: the largest power of two with G * pieces
+// <= 64).
+__device__ __forceinline__ void pr_tail_body(const PrSweep& P, const PrTailDev& Z) {
+    __shared__ float s_stage[kTailWaves][kWaveNnz];
+    const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+    const uint32_t it = blockIdx.x * kTailWaves + w;
+    if (it >= Z.n_items) return;  // wave-uniform; no workgroup barrier follows
+    const uint2 item = Z.items[it];
+    const uint32_t x0 = Z.px[item.x], cnt = Z.px[item.y] - x0;
+    float* st = s_stage[w];
+    for (uint32_t base = 0; base < cnt; base += kGatherBatch * kWave) {
+        uint32_t u[kGatherBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            const uint32_t i = base + j * kWave + lane;
+            u[j] = i < cnt ? __builtin_nontemporal_load(&Z.tci[x0 + i]) : 0u;
+        }
+        float v[kGatherBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            const uint32_t i = base + j * kWave + lane;
+            v[j] = i < cnt ? P.contrib_in[u[j]] : 0.0f;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            const uint32_t i = base + j * kWave + lane;
+            if (i < cnt) st[i] = v[j];
+        }
+    }
+    wave_lds_sync();
+    const uint32_t pieces = item.y - item.x;
+    uint32_t G = pieces >= kWave ? 1u : kWave / pieces;
+    G = G >= 64 ? 64 : (G >= 32 ? 32 : (G >= 16 ? 16 : (G >= 8 ? 8 : (G >= 4 ? 4 : (G >= 2 ? 2 : 1)))));
+    const uint32_t per_round = kWave / G, rr = lane / G, gl = lane % G;
+    for (uint32_t pb = 0; pb < pieces; pb += per_round) {
+        const uint32_t q = item.x + pb + rr;
+        Kahan k;
+        const bool ok = pb + rr < pieces;
+        if (ok) {
+            const uint32_t a = Z.px[q] - x0, b = Z.px[q + 1] - x0;
+            for (uint32_t x = a + gl; x < b; x += G) k.add(st[x]);
+        }
+        float hi = k.s, lo = -k.c;
+        comp_reduce(hi, lo, G);
+        if (ok && gl == 0) Z.part[Z.pdst[q]] = make_float2(hi, lo);
+    }
+}
+
+// Adds one (hi, lo) partial to a running TwoSum pair.
+__device__ __forceinline__ void tail_add(float& H, float& L, float2 q) {
+    float a, e;
+    two_sum(H, q.x, a, e);
+    H = a;
+    L = L + q.y + e;
+}
+
+// Head chunk partials of split row j: lpart[first, first + count).
+__device__ __forceinline__ uint2 tail_head(const PrSweep& P, const PrTailDev& Z, uint32_t j) {
+    const uint32_t lr = Z.slr[j];
+    if (lr == kNoHead) return make_uint2(0, 0);
+    const uint2 e = P.stream.lrows[lr];
+    return make_uint2(e.x, e.y & ~kLrowSplit);
+}
+
+// Split rows whose head and overflow partials number more than
+// kMergeLanePartials (the list `big`) get a wave each: lane l adds partials l,
+// l + 64, ... of the row's sequence (head chunks, tiles, overflow slots), then a
+// TwoSum butterfly; every other split row gets a lane that adds its sequence in
+// order.
+__device__ __forceinline__ void pr_tail_merge_body(const PrSweep& P, const PrTailDev& Z) {
+    const double addd = (1.0 - (double)P.damping) / (double)P.n + (double)P.damping * P.acc_in->dangling / (double)P.n;
+    const float add = (float)addd, d = P.damping, inv_n = 1.0f / (float)P.n;
+    const uint32_t lane = lane_id();
+    const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nw = gridDim.x * blockDim.x / kWave;
+    PrPart part;
+    for (uint32_t i = gw; i < Z.n_big; i += nw) {
+        const uint32_t j = Z.big[i];
+        const uint2 h = tail_head(P, Z, j);
+        const uint32_t o0 = Z.ovptr[j], no = Z.ovptr[j + 1] - o0;
+        const uint32_t total = h.y + Z.T + no;
+        float H = 0.f, L = 0.f;
+        for (uint32_t q = lane; q < total; q += kWave) {
+            float2 x;
+            if (q < h.y) x = P.stream.lpart[h.x + q];
+            else if (q < h.y + Z.T) x = Z.part[(size_t)(q - h.y) * Z.R + j];
+            else x = Z.part[(size_t)Z.T * Z.R + o0 + (q - h.y - Z.T)];
+            tail_add(H, L, x);
+        }
+        comp_reduce(H, L, kWave);
+        if (lane == 0) pr_store(P, Z.srow[j], add + d * (H + L), part, inv_n);
+    }
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < Z.R; j += gridDim.x * blockDim.x) {
+        const uint2 h = tail_head(P, Z, j);
+        const uint32_t o0 = Z.ovptr[j], o1 = Z.ovptr[j + 1];
+        if (h.y + (o1 - o0) > kMergeLanePartials) continue;  // in `big`
+        float H = 0.f, L = 0.f;
+        for (uint32_t q = 0; q < h.y; ++q) tail_add(H, L, P.stream.lpart[h.x + q]);
+        for (uint32_t t = 0; t < Z.T; ++t) tail_add(H, L, Z.part[(size_t)t * Z.R + j]);
+        for (uint32_t o = o0; o < o1; ++o) tail_add(H, L, Z.part[(size_t)Z.T * Z.R + o]);
+        pr_store(P, Z.srow[j], add + d * (H + L), part, inv_n);
+    }
+    write_block_partials(part, P.partial, P.partial_cap, P.partial_base + blockIdx.x);
+}
+
+}  // namespace ggpu

@@ -307,6 +307,7 @@ void ggpu_check_ranks(uint64_t n, const double* ref, const float* got, double* l
 
 // ---- test hooks (include/ggpu_testing.h)
 #include "ggpu_testing.h"
+#include "pr_tail_plan.hpp"
 #include "pr_tiling.hpp"
 
 extern "C" int ggpu_test_pr_tiled_emulate(uint64_t n, uint64_t m, const uint32_t* in_row_ptr, const uint32_t* in_col_idx,
@@ -334,5 +335,29 @@ extern "C" int ggpu_test_pr_tiled_emulate(uint64_t n, uint64_t m, const uint32_t
         stats[5] = plan.items.size();
         stats[6] = plan.tasks.size();
         stats[7] = plan.segs.size();
+    });
+}
+
+extern "C" int ggpu_test_pr_tail_emulate(uint64_t n, uint64_t m, const uint32_t* in_row_ptr, const uint32_t* in_col_idx,
+                                         const uint32_t* params, const float* contrib, double* sums, uint64_t* stats) {
+    if (!in_row_ptr || (m && !in_col_idx) || !params || !contrib || !sums || !stats) return fail(GGPU_EINVAL, "null argument");
+    return guarded([&] {
+        const ggpu::Csr in = ggpu::from_arrays(n, m, in_row_ptr, in_col_idx, GGPU_FLAG_SORTED);
+        ggpu::TailParams p;
+        p.min_indeg = params[0];
+        p.tail_begin = params[1];
+        p.tile = params[2];
+        p.nnz_cap = params[3];
+        p.pieces_cap = params[4];
+        const ggpu::TailPlan plan = ggpu::plan_tail(in, p);
+        ggpu::emulate_tail(in, p, plan, contrib, sums);
+        stats[0] = plan.R;
+        stats[1] = plan.T;
+        stats[2] = plan.tci.size();
+        stats[3] = plan.head_arcs;
+        stats[4] = plan.pdst.size();
+        stats[5] = plan.items.size();
+        stats[6] = plan.ovptr.back();
+        stats[7] = 0;
     });
 }

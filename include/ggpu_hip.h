@@ -194,6 +194,13 @@ typedef struct ggpu_pr_opts {
  * the other rows as in HOT. Built for ORDER_DEGREE (hub rows and hot sources
  * first). Jacobi sweeps only: in-place sweeps with this layout use STREAM. */
 #define GGPU_PR_LAYOUT_TILED 3
+/* HOT_TAIL: HOT, except that rows of in-degree >= large_min leave their in-arcs
+ * from tail sources (ids >= 2^19, past the LDS copy and an L2-sized band) to a
+ * second kernel that sums them per tile of 2^19 sources in tile order (the tile
+ * being read stays in L2), one compensated partial per (tile, row); a merge adds
+ * each such row's head, tile and overflow partials in a fixed order. Built for
+ * ORDER_DEGREE. Jacobi sweeps only: in-place sweeps with this layout use STREAM. */
+#define GGPU_PR_LAYOUT_HOT_TAIL 4
 void ggpu_pr_opts_default(ggpu_pr_opts* o);
 /* "gpu_pull_scalar" | "gpu_pull_vector" | "gpu_pull_adaptive" | "gpu_pull_gs" | "gpu_push" -> GGPU_PR_* */
 int ggpu_pr_parse_variant(const char* name, int* variant);

@@ -29,6 +29,17 @@ int ggpu_test_scan(ggpu_ctx* ctx, uint32_t n, const uint32_t* in, uint32_t* out,
 int ggpu_test_frontier_convert(ggpu_bfs* b, const uint32_t* dist, uint32_t level, uint32_t tg, uint64_t* bitmap,
                                uint32_t* queue, uint32_t* degrees, uint32_t* count);
 
+/* Sets the HOT_TAIL geometry of p's later layouts: the first tail source and
+ * the sources per tail tile (0: the defaults, 2^19 and 2^19), so that small
+ * graphs run tails of many tiles. A layout built with other values is rebuilt
+ * on its next run. */
+int ggpu_test_pr_tail_geometry(ggpu_pr* p, uint32_t tail_begin, uint32_t tile);
+
+/* The HOT_TAIL layout last built for vertex order `order` (GGPU_PR_ORDER_*):
+ * stats (6 words) = split rows, tail tiles, tail arcs, tail items, split rows
+ * merged by a wave, long-row chunks of the HOT items; all zero if none. */
+int ggpu_test_pr_tail_info(const ggpu_pr* p, uint32_t order, uint64_t* stats);
+
 #ifdef __cplusplus
 }
 #endif

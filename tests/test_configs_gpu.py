@@ -153,12 +153,13 @@ def test_rmat23_c5_certified_jacobi():
     conv = _pr64(rp, ci, od, 1000, 1e-10)
     pr = G.GpuPageRank(g, od)
     tol_cert = 1e-4 * 0.15 / 0.85  # ||r_k - r*||_1 <= d / (1 - d) * delta_k
-    for layout in (G.PR_LAYOUT_HOT, G.PR_LAYOUT_STREAM):
+    for layout, lmin in ((G.PR_LAYOUT_HOT, 256), (G.PR_LAYOUT_STREAM, 256), (G.PR_LAYOUT_HOT_TAIL, 64),
+                         (G.PR_LAYOUT_HOT_TAIL, 256)):
         res = pr.run("gpu_pull_adaptive", G.pr_options(tol=tol_cert, iters=1000, order=G.PR_ORDER_DEGREE,
-                                                       layout=layout))
+                                                       layout=layout, large_min=lmin))
         l1, rel, s = O.check_ranks(conv, pr.ranks())
-        assert not res.floor_stop and res.last_delta < tol_cert, (layout, res.last_delta)
-        assert l1 < 1e-4 and rel < 1e-3 and abs(s - 1) < 1e-4, (layout, l1, rel, s)
+        assert not res.floor_stop and res.last_delta < tol_cert, (layout, lmin, res.last_delta)
+        assert l1 < 1e-4 and rel < 1e-3 and abs(s - 1) < 1e-4, (layout, lmin, l1, rel, s)
     # in-place Gauss-Seidel to tol 1e-6: within the reference's acceptance of the
     # converged ranks, and no worse than FP64 Jacobi stopped at the same tolerance
     for order in (G.PR_ORDER_DEGREE, G.PR_ORDER_GIVEN):
