@@ -92,7 +92,8 @@ __device__ __forceinline__ T block_sum(T v, T* scratch) {
     do {                                                                                          \
         hipError_t e_ = (expr);                                                                   \
         if (e_ != hipSuccess)                                                                     \
-            throw ::ggpu::DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_), e_);     \
+            throw ::ggpu::DeviceError(std::string(#expr) + " (" + __FILE__ + ":" +                \
+                                      std::to_string(__LINE__) + "): " + hipGetErrorString(e_), e_); \
     } while (0)
 
 struct DeviceError {

@@ -44,6 +44,7 @@
 #include <cstring>
 
 #include "bfs_engine.hpp"
+#include "ggpu_hip_testing.h"
 #include "grid_sync.hpp"
 
 namespace ggpu {
@@ -89,6 +90,27 @@ constexpr uint32_t kPartWords = 4;
 constexpr uint32_t kMsHotWords = 16384;  // 128 KiB of LDS
 constexpr uint32_t kMsHotBit = 1u << 31;
 constexpr uint32_t kMsZeroWords = 64;
+// The kernel's big LDS buffer: the bottom-up hot words, or the routed
+// top-down's queue prefix and bin counters, then its next words of one bin.
+__shared__ uint64_t g_ms_lds[kMsHotWords];
+
+// Routed top-down (ms_route_*): a top-down level whose frontier has at most
+// kRouteQ vertices and at least kRouteMinEdges arcs. Records are (vertex - bin
+// start) << kRouteQBits | queue index; a bin is a word-aligned vertex range of
+// at most kRouteBinV / W vertices (its next words fill the LDS buffer).
+constexpr uint32_t kRouteQ = 4096;
+constexpr uint32_t kRouteQBits = 12;
+constexpr uint32_t kRouteBinV = kMsHotWords;
+constexpr uint32_t kRouteBins = 2048;
+constexpr uint32_t kRouteArcs = 8;                        // consecutive arcs a thread bins per round
+constexpr uint32_t kRouteChunk = kRouteArcs * kMsThreads;  // arcs a workgroup bins per round
+constexpr uint64_t kRouteMinEdges = 65536;
+static_assert((kRouteBinV << kRouteQBits) <= (1ull << 32) && kRouteQ <= (1u << kRouteQBits), "record fields");
+// Abort code of a routed level whose queue, arc total or bin reservations do not
+// match the level's counters (the phases then skip their writes; every
+// workgroup leaves at the next barrier and the host reports it).
+constexpr uint32_t kAbortRoute = 2;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
 struct MsDev {
     const uint32_t* rp;
@@ -123,6 +145,17 @@ struct MsDev {
     uint32_t n, k, nb, n_heavy, C, trace_cap, n_actions, policy;
     uint32_t n_hot;                // <= kMsHotWords / W
     uint32_t heavy_min;            // degree from which a vertex is in the heavy item list
+    // routed top-down (n_bins == 0: not available)
+    uint32_t* rq;                  // kRouteQ frontier vertices
+    uint32_t* rqd;                 // their degrees
+    uint64_t* rqf;                 // their frontier words, kRouteQ * W
+    uint32_t* rctl;                // [0] queue reservations, [1 + b] records of bin b (zero between levels)
+    const uint32_t* blo;           // n_bins + 1 bin starts (multiples of 64; the last is nb * 64)
+    const uint32_t* bbase;         // n_bins record-area starts (area b: the in-degree sum of bin b)
+    const uint16_t* wbin;          // nb: the bin of each 64-vertex word
+    uint32_t* recs;                // records
+    uint32_t n_bins;
+    uint64_t route_min;            // frontier arcs from which a small-frontier top-down level is routed
 };
 
 struct MsAcc {
@@ -237,7 +270,7 @@ __device__ void ms_bottom_up(const MsDev& D, uint32_t level, bool use_hot, bool 
     const uint32_t lane = lane_id();
     // this level's frontier words of the hot vertices (zero: not in the frontier);
     // a level with little left to scan does not pay for the copy and reads col_idx
-    __shared__ uint64_t s_hot[kMsHotWords];
+    auto& s_hot = g_ms_lds;
     const uint32_t* cx = use_hot ? D.cib : D.ci;
     if (use_hot) {
         for (uint32_t i = threadIdx.x; i < D.n_hot * W; i += blockDim.x)
@@ -586,6 +619,248 @@ __device__ void ms_td_push(const MsDev& D, uint32_t level, bool lane_per_vertex)
             ms_push_range<W, kMark, kFilter>(D, fbn, y, z, lane, kWave, f);
         }
     }
+}
+
+// ------------------------------------------------------- routed top-down
+// A top-down level with a small frontier of hubs (the level after the sources:
+// 1454 vertices, 2.9M arcs on rmat22_c2) spends its time in device-scope
+// atomicOr pushes, which execute beyond the XCD's L2. Routed, the same level
+// keeps every atomic in LDS:
+//  1. queue (ms_route_queue): the frontier's vertices, degrees and words, one
+//     reservation per wave; grid barrier;
+//  2. bin (ms_route_bin): every workgroup takes an equal slice of the
+//     frontier's arcs in queue order and, per round of kRouteChunk arcs,
+//     counts its records per bin in LDS, reserves each bin's run with one
+//     device atomic and writes the records there (write-through); grid barrier;
+//  3. apply (ms_route_apply): a workgroup takes whole bins, ORs each record's
+//     frontier words into an LDS copy of the bin's next words, then settles the
+//     bin's vertices (new bits = next & ~visit) as the collect does.
+// A bin's record area holds the in-degree sum of its vertices, the most
+// records one level can put there. Discoveries equal the pushed level's.
+template <int W>
+__device__ void ms_route_queue(const MsDev& D, uint32_t level) {
+    const uint32_t cs = level % D.C;
+    const uint64_t* F = D.F + (size_t)cs * D.n * W;
+    const uint64_t* fb = D.fb + (size_t)cs * D.nb;
+    const uint32_t lane = lane_id(), nw = nwaves();
+    for (uint64_t r0 = 0; r0 < D.nb; r0 += (uint64_t)nw * kWave) {
+        const uint32_t wi = ms_word_of(r0, lane, nw);
+        const uint64_t fw = wi < D.nb ? fb[wi] : 0ull;
+        // the wave takes the nonzero words one at a time, a lane per bit (the
+        // hubs of the degree numbering fill whole words)
+        uint64_t todo = __ballot(fw != 0);
+        while (todo) {
+            const int j = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            const uint64_t word = __shfl(fw, j, kWave);
+            const uint32_t wj = __shfl(wi, j, kWave);
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&D.rctl[0], (uint32_t)__popcll(word));
+            base = __shfl(base, 0, kWave);
+            if (base + (uint32_t)__popcll(word) > kRouteQ) {  // more frontier vertices than counted
+                if (lane == 0) atomicExch(D.abort, kAbortRoute);
+                continue;
+            }
+            if ((word >> lane) & 1ull) {
+                const uint32_t q = base + (uint32_t)__popcll(word & ((1ull << lane) - 1ull));
+                const uint32_t v = wj * 64 + lane;
+                store_through(&D.rq[q], v);
+                store_through(&D.rqd[q], D.rp[v + 1] - D.rp[v]);
+#pragma unroll
+                for (int w = 0; w < W; ++w) store_through(&D.rqf[(size_t)q * W + w], F[(size_t)v * W + w]);
+            }
+        }
+    }
+}
+
+// Exclusive prefix of x[0, count) in place (count <= 8 * blockDim.x); x[count] = total.
+__device__ __forceinline__ void block_exclusive_scan(uint32_t* x, uint32_t count) {
+    __shared__ uint32_t s_wsum[kMsThreads / kWave];
+    const uint32_t t = threadIdx.x, per = 8, lane = lane_id(), w = t / kWave;
+    uint32_t v[8], sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < per; ++j) {
+        const uint32_t i = t * per + j;
+        v[j] = i < count ? x[i] : 0u;
+        sum += v[j];
+    }
+    const uint32_t incl = wave_inclusive_scan(sum);
+    if (lane == kWave - 1) s_wsum[w] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t j = 0; j < w; ++j) before += s_wsum[j];
+    uint32_t run = before + incl - sum;
+#pragma unroll
+    for (uint32_t j = 0; j < per; ++j) {
+        const uint32_t i = t * per + j;
+        if (i < count) x[i] = run;
+        run += v[j];
+    }
+    if (t == blockDim.x - 1) x[count] = run;  // the last thread's run is the total
+    __syncthreads();
+}
+
+template <int W>
+__device__ void ms_route_bin(const MsDev& D, uint32_t S, uint64_t E) {
+    uint32_t* sD = (uint32_t*)g_ms_lds;   // S + 1 prefix of the queue's degrees
+    uint32_t* sR = sD + kRouteQ + 1;      // S row starts of the queue's vertices
+    uint32_t* sCnt = sR + kRouteQ;        // per bin: records of this round, then their LDS offsets
+    uint32_t* sGb = sCnt + kRouteBins + 1;  // per bin: where this round's run starts in the bin's area
+    uint32_t* sBlo = sGb + kRouteBins;    // n_bins + 1 bin starts
+    uint32_t* sRec = sBlo + kRouteBins + 1;  // the round's records, by bin
+    uint32_t* sDst = sRec + kRouteChunk;     // and where each goes
+    const uint32_t t = threadIdx.x, T = blockDim.x, NB = D.n_bins;
+    // the queue was written with write-through stores and the barrier's acquire
+    // has invalidated stale lines: plain loads from here on
+    for (uint32_t i = t; i < S; i += T) {
+        const uint32_t v = D.rq[i];
+        sD[i] = D.rqd[i];
+        sR[i] = D.rp[v];
+    }
+    for (uint32_t b = t; b <= NB; b += T) sBlo[b] = D.blo[b];
+    for (uint32_t b = t; b <= NB; b += T) sCnt[b] = 0;
+    __syncthreads();
+    block_exclusive_scan(sD, S);
+    if ((uint64_t)sD[S] != E) {  // the queue's arcs are not the counted frontier arcs
+        if (t == 0) atomicExch(D.abort, kAbortRoute);
+        return;  // workgroup-uniform: sD[S] and E are the same in every thread
+    }
+    const uint64_t s0 = E * blockIdx.x / gridDim.x, s1 = E * (blockIdx.x + 1) / gridDim.x;
+    for (uint64_t c0 = s0; c0 < s1; c0 += kRouteChunk) {  // rounds are uniform per workgroup
+        // thread t takes arcs [i0, i0 + kRouteArcs): one search for the queue entry
+        // holding i0 (the last k with sD[k] <= i0), then a walk across entries
+        const uint64_t i0 = c0 + (uint64_t)t * kRouteArcs;
+        uint32_t k = 0;
+        if (i0 < s1) {
+            uint32_t hi = S;
+            while (hi - k > 1) {
+                const uint32_t mid = (k + hi) >> 1;
+                if (sD[mid] <= (uint32_t)i0) k = mid;
+                else hi = mid;
+            }
+        }
+        uint32_t u[kRouteArcs], kk[kRouteArcs];
+        bool ok[kRouteArcs];
+#pragma unroll
+        for (uint32_t j = 0; j < kRouteArcs; ++j) {
+            const uint32_t i = (uint32_t)(i0 + j);
+            ok[j] = i0 + j < s1;
+            if (ok[j])
+                while (sD[k + 1] <= i) ++k;  // sD[S] = E > i: stops inside the queue
+            kk[j] = k;
+            u[j] = ok[j] ? D.ci[sR[k] + (i - sD[k])] : 0u;
+        }
+        uint32_t rec[kRouteArcs], bin[kRouteArcs], slot[kRouteArcs];
+#pragma unroll
+        for (uint32_t j = 0; j < kRouteArcs; ++j) bin[j] = ok[j] ? D.wbin[u[j] >> 6] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < kRouteArcs; ++j) {
+            rec[j] = ((u[j] - sBlo[bin[j]]) << kRouteQBits) | kk[j];
+            slot[j] = ok[j] ? atomicAdd(&sCnt[bin[j]], 1u) : 0u;
+        }
+        __syncthreads();
+        for (uint32_t b = t; b < NB; b += T) {
+            const uint32_t c = sCnt[b];
+            uint32_t g = 0;
+            if (c) {
+                const uint32_t off = atomicAdd(&D.rctl[1 + b], c);
+                g = D.bbase[b] + off;
+                if ((uint64_t)off + c > (uint64_t)(D.bbase[b + 1] - D.bbase[b])) {  // past the bin's area
+                    atomicExch(D.abort, kAbortRoute);
+                    g = kNoSlot;
+                }
+            }
+            sGb[b] = g;
+        }
+        block_exclusive_scan(sCnt, NB);  // counts -> LDS offsets of the bins' runs
+        const uint32_t total = sCnt[NB];  // read before any thread clears sCnt at the round's end
+#pragma unroll
+        for (uint32_t j = 0; j < kRouteArcs; ++j)
+            if (ok[j]) {
+                const uint32_t p = sCnt[bin[j]] + slot[j];
+                const uint32_t g = sGb[bin[j]];
+                sRec[p] = rec[j];
+                sDst[p] = g == kNoSlot ? kNoSlot : g + slot[j];
+            }
+        __syncthreads();
+        // a bin's run is consecutive in LDS and in its area: consecutive lanes write consecutive words
+        for (uint32_t p = t; p < total; p += T)
+            if (sDst[p] != kNoSlot) store_through(&D.recs[sDst[p]], sRec[p]);
+        for (uint32_t b = t; b <= NB; b += T) sCnt[b] = 0;
+        __syncthreads();
+    }
+}
+
+template <int W>
+__device__ void ms_route_apply(const MsDev& D, uint32_t level, uint32_t S, MsAcc& a) {
+    __shared__ uint32_t s_rc;
+    uint64_t* sNext = g_ms_lds;
+    const uint32_t ns = (level + 1) % D.C;
+    uint64_t* Fn = D.F + (size_t)ns * D.n * W;
+    uint64_t* fbn = D.fb + (size_t)ns * D.nb;
+    const uint32_t t = threadIdx.x, T = blockDim.x, lane = lane_id(), wv = t / kWave, nwv = T / kWave;
+    for (uint32_t b = blockIdx.x; b < D.n_bins; b += gridDim.x) {
+        if (t == 0) s_rc = load_coherent(&D.rctl[1 + b]);  // device atomics made it
+        __syncthreads();
+        const uint32_t cnt = s_rc;
+        if (cnt) {  // workgroup-uniform
+            const uint32_t lo = D.blo[b], hi = D.blo[b + 1];
+            for (uint32_t x = t; x < (hi - lo) * W; x += T) sNext[x] = 0;
+            __syncthreads();
+            const uint32_t* rec = D.recs + D.bbase[b];
+            for (uint32_t i = t; i < cnt; i += T) {
+                // written through before the barrier, whose acquire dropped stale lines: plain loads
+                const uint32_t r = rec[i];
+                const uint32_t off = r >> kRouteQBits, q = r & (kRouteQ - 1);
+                if (off >= hi - lo || q >= S) continue;  // not a record of this level (never with matching counters)
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    const uint64_t f = D.rqf[(size_t)q * W + w];
+                    if (f) atomicOr((unsigned long long*)&sNext[(size_t)off * W + w], (unsigned long long)f);
+                }
+            }
+            __syncthreads();
+            // settle [lo, hi): a wave per 64-vertex word, as ms_td_collect_all
+            for (uint32_t u0 = lo + wv * kWave; u0 < hi; u0 += nwv * kWave) {
+                const uint32_t u = u0 + lane;
+                uint32_t pc = 0;
+                bool full = u < D.n;
+                if (u < D.n) {
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        const size_t i = (size_t)u * W + w;
+                        const uint64_t nx = sNext[(size_t)(u - lo) * W + w];
+                        full &= nx != 0;
+                        if (nx) {
+                            const uint64_t vis = D.visit[i];
+                            const uint64_t nwb = nx & ~vis;
+                            full &= (vis | nwb) == valid_word<W>(D, w);
+                            if (nwb) {
+                                store_through(&Fn[i], nwb);
+                                store_through(&D.visit[i], vis | nwb);
+                                pc += (uint32_t)__popcll(nwb);
+                            }
+                        }
+                    }
+                }
+                const uint64_t found = __ballot(pc != 0);
+                const uint64_t cmask = __ballot(pc != 0 && full);
+                if (lane == 0 && cmask) atomicOr((unsigned long long*)&D.done[u0 >> 6], (unsigned long long)cmask);
+                if (pc) {
+                    const uint64_t dg = D.rp[u + 1] - D.rp[u];
+                    a.pairs += dg * pc;
+                    a.edges += dg;
+                }
+                if (lane == 0 && found) {
+                    store_through(&fbn[u0 >> 6], found);
+                    a.size += (uint64_t)__popcll(found);
+                }
+            }
+            if (t == 0) store_through(&D.rctl[1 + b], 0u);  // the bin is empty for the next routed level
+        }
+        __syncthreads();  // s_rc and the LDS words are reused by the next bin
+    }
+    if (blockIdx.x == 0 && t == 0) store_through(&D.rctl[0], 0u);  // the queue, likewise
 }
 
 // Settles marked vertex u: keeps next & ~visit as its new bits, clears next.
@@ -1000,12 +1275,22 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
             // memory operations than one contended bitmap atomic per push)
             const bool scan_all = (uint64_t)t.edges * 8 > (uint64_t)D.n;
             const bool binned = act == GGPU_ACTION_TD_BINNED;
-            if (!scan_all) ms_td_push<W, true, true>(D, level, binned);
-            else if (pairs_u * 2 > total_pairs) ms_td_push<W, false, false>(D, level, binned);
-            else ms_td_push<W, false, true>(D, level, binned);
-            if (!grid_barrier(D.bar, D.abort)) return;
-            if (scan_all) ms_td_collect_all<W>(D, level, a);
-            else ms_td_collect<W>(D, level, a);
+            if (D.n_bins && t.size <= kRouteQ && t.edges >= D.route_min) {
+                // a small frontier with many arcs: routed through LDS bins
+                if (blockIdx.x == 0 && threadIdx.x == 0) D.rctl[1 + kRouteBins]++;  // routed levels of this run
+                ms_route_queue<W>(D, level);
+                if (!grid_barrier(D.bar, D.abort)) return;
+                ms_route_bin<W>(D, (uint32_t)t.size, t.edges);
+                if (!grid_barrier(D.bar, D.abort)) return;
+                ms_route_apply<W>(D, level, (uint32_t)t.size, a);
+            } else {
+                if (!scan_all) ms_td_push<W, true, true>(D, level, binned);
+                else if (pairs_u * 2 > total_pairs) ms_td_push<W, false, false>(D, level, binned);
+                else ms_td_push<W, false, true>(D, level, binned);
+                if (!grid_barrier(D.bar, D.abort)) return;
+                if (scan_all) ms_td_collect_all<W>(D, level, a);
+                else ms_td_collect<W>(D, level, a);
+            }
         }
         ms_flush(D, (level + 1) & 1, a);
         if (!grid_barrier(D.bar, D.abort)) return;
@@ -1054,6 +1339,12 @@ struct MsState {
     DBuf<uint8_t> actions;
     uint32_t n_heavy = 0;
     bool heavy_ready = false;
+    // routed top-down: bins of the traversal numbering for route_w words per vertex
+    uint32_t route_w = 0, n_bins = 0;
+    uint64_t route_min = kRouteMinEdges;  // test hook override
+    DBuf<uint32_t> rq, rqd, rctl, blo, bbase, recs;
+    DBuf<uint64_t> rqf;
+    DBuf<uint16_t> wbin;
     uint32_t W = 0, C = 0, k = 0;
     uint32_t grid[5] = {0, 0, 0, 0, 0};
     bool clean = false;  // visit / next / F / fb all zero
@@ -1164,6 +1455,7 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         if (S.order != order) {
             S.heavy_ready = false;
             S.hot_w = 0;
+            S.route_w = 0;
             S.relabeled = Csr();
             S.perm_host.clear();
             S.rp2.release();
@@ -1236,6 +1528,59 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             S.n_hot = H;
             S.hot_w = W;
         }
+        if (S.route_w != W) {
+            // bins of the routed top-down: word-aligned vertex ranges of at most
+            // kRouteBinV / W vertices; a bin also closes before a word that would take
+            // its in-degree sum past cap_d (m / 384 to start with, so that bins carry
+            // similar record counts and the hub words get bins of their own; doubled
+            // until there are at most kRouteBins). Bin b's record area is its in-degree sum.
+            std::vector<uint64_t> wdeg(nb, 0);
+            for (const uint32_t u : hg.ci) wdeg[u >> 6]++;
+            const uint64_t cap_v = kRouteBinV / W;
+            std::vector<uint32_t> blo;
+            std::vector<uint64_t> bdeg;
+            std::vector<uint16_t> wbin(std::max<uint32_t>(nb, 1), 0);
+            for (uint64_t cap_d = std::max<uint64_t>(e.m / 384, 1);; cap_d *= 2) {
+                blo.assign(1, 0);
+                bdeg.clear();
+                uint64_t cur_v = 0, cur_d = 0;
+                for (uint32_t wd = 0; wd < nb; ++wd) {
+                    if (cur_v && (cur_v + 64 > cap_v || cur_d + wdeg[wd] > cap_d)) {
+                        bdeg.push_back(cur_d);
+                        blo.push_back(wd * 64);
+                        cur_v = cur_d = 0;
+                    }
+                    cur_v += 64;
+                    cur_d += wdeg[wd];
+                    wbin[wd] = (uint16_t)std::min<size_t>(blo.size() - 1, 0xFFFF);
+                }
+                bdeg.push_back(cur_d);
+                blo.push_back(nb * 64);
+                if (blo.size() - 1 <= kRouteBins || cap_d > e.m) break;
+            }
+            const uint32_t NB = blo.size() - 1 <= kRouteBins ? (uint32_t)(blo.size() - 1) : 0u;
+            std::vector<uint32_t> bbase(bdeg.size() + 1);  // + the end of the last area
+            uint64_t acc = 0;
+            for (size_t b = 0; b < bdeg.size(); ++b) {
+                bbase[b] = (uint32_t)acc;
+                acc += bdeg[b];
+            }
+            bbase[bdeg.size()] = (uint32_t)acc;
+            S.blo.alloc(blo.size());
+            S.bbase.alloc(bbase.size());
+            S.wbin.alloc(wbin.size());
+            GGPU_HIP_TRY(hipMemcpy(S.blo.p, blo.data(), blo.size() * 4, hipMemcpyHostToDevice));
+            GGPU_HIP_TRY(hipMemcpy(S.bbase.p, bbase.data(), bbase.size() * 4, hipMemcpyHostToDevice));
+            GGPU_HIP_TRY(hipMemcpy(S.wbin.p, wbin.data(), wbin.size() * 2, hipMemcpyHostToDevice));
+            S.recs.alloc(std::max<uint64_t>(e.m, 1));
+            S.rq.alloc(kRouteQ);
+            S.rqd.alloc(kRouteQ);
+            S.rqf.alloc((size_t)kRouteQ * 4);
+            S.rctl.alloc(2 + kRouteBins);  // + the routed-level count of the last run
+            GGPU_HIP_TRY(hipMemset(S.rctl.p, 0, (2 + kRouteBins) * sizeof(uint32_t)));
+            S.n_bins = NB;
+            S.route_w = W;
+        }
         // level slots: kMsSlots, fewer if device memory is short (at least 2)
         if (S.W != W) {
             S.visit.release();
@@ -1264,6 +1609,7 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             GGPU_HIP_TRY(hipMemsetAsync(S.F.p, 0, (size_t)S.C * words * 8, s));
             GGPU_HIP_TRY(hipMemsetAsync(S.fb.p, 0, (size_t)S.C * nb * 8, s));
             if (S.st.p) GGPU_HIP_TRY(hipMemsetAsync(S.st.p, 0, kMsState * 8, s));
+            if (S.rctl.p) GGPU_HIP_TRY(hipMemsetAsync(S.rctl.p, 0, (2 + kRouteBins) * sizeof(uint32_t), s));
             S.clean = true;
         }
         if (!S.zero.p) {
@@ -1306,6 +1652,17 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
         D.hot = S.hot.p;
         D.n_hot = S.n_hot;
         D.heavy_min = heavy_min;
+        D.rq = S.rq.p;
+        D.rqd = S.rqd.p;
+        D.rqf = S.rqf.p;
+        D.rctl = S.rctl.p;
+        D.blo = S.blo.p;
+        D.bbase = S.bbase.p;
+        D.wbin = S.wbin.p;
+        D.recs = S.recs.p;
+        D.n_bins = S.n_bins;
+        D.route_min = S.route_min;
+        GGPU_HIP_TRY(hipMemsetAsync(S.rctl.p + 1 + kRouteBins, 0, sizeof(uint32_t), s));
         D.visit = S.visit.p;
         D.next = S.next.p;
         D.F = S.F.p;
@@ -1362,6 +1719,9 @@ int ggpu_msbfs_run(ggpu_bfs* b, const uint32_t* sources, uint32_t k, const ggpu_
             GGPU_HIP_TRY(hipEventRecord(S.ev[1], s));  // the last launch's record is the one timed
             GGPU_HIP_TRY(hipMemcpyAsync(ctl, S.ctl.p + kBarWords, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             GGPU_HIP_TRY(hipStreamSynchronize(s));
+            if (ctl[0] == kAbortRoute)
+                throw ArgError("batched BFS: a routed top-down level's queue or bins disagree with its counters",
+                               GGPU_EDEVICE);
             if (ctl[0]) throw ArgError("batched BFS: grid barrier timed out", GGPU_ETIMEOUT);
             float fms = 0;
             GGPU_HIP_TRY(hipEventElapsedTime(&fms, S.ev[2], S.ev[3]));
@@ -1433,6 +1793,23 @@ int ggpu_msbfs_record_levels(ggpu_bfs* b, const uint32_t* sources, uint32_t k, c
             if (trace) trace[l] = runs[0][l];
         }
         if (len) *len = levels;
+    });
+}
+
+int ggpu_test_msbfs_route(ggpu_bfs* b, uint64_t min_edges, uint32_t* routed_last) {
+    BfsEngine* ep = ggpu_bfs_engine(b);
+    if (!ep) return GGPU_EINVAL;
+    return guarded(ep->ctx, [&] {
+        MsState& S = ms_state(*ep);
+        S.route_min = min_edges ? min_edges : kRouteMinEdges;
+        if (routed_last) {
+            *routed_last = 0;
+            if (S.rctl.p) {
+                GGPU_HIP_TRY(hipSetDevice(ep->ctx->device));
+                GGPU_HIP_TRY(hipStreamSynchronize(ep->ctx->stream));
+                GGPU_HIP_TRY(hipMemcpy(routed_last, S.rctl.p + 1 + kRouteBins, sizeof(uint32_t), hipMemcpyDeviceToHost));
+            }
+        }
     });
 }
 

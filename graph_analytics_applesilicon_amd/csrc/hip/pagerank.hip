@@ -189,7 +189,7 @@ struct TailLayout {
     uint32_t min_indeg = 0, tail_begin = 0, tile = 0;
     StreamLayout stream;
     uint32_t R = 0, T = 0, n_items = 0, n_big = 0;
-    uint64_t tail_arcs = 0;
+    uint64_t tail_arcs = 0, n_part = 0;
     DBuf<uint32_t> tci, px, pdst, srow, slr, ovptr, big;
     DBuf<uint2> items;
     DBuf<float2> part;
@@ -201,6 +201,7 @@ struct TailLayout {
         Z.items = items.p;
         Z.n_items = n_items;
         Z.part = part.p;
+        Z.n_part = n_part;
         Z.srow = srow.p;
         Z.slr = slr.p;
         Z.ovptr = ovptr.p;
@@ -572,6 +573,7 @@ struct PrEngine {
         upload_vec(T->big, big);
         // tile partials of rows without arcs in a tile stay zero: cleared once here
         const size_t np = (size_t)plan.T * plan.R + plan.ovptr.back();
+        T->n_part = np;
         T->part.alloc(std::max<size_t>(np, 1));
         GGPU_HIP_TRY(hipMemset(T->part.p, 0, std::max<size_t>(np, 1) * sizeof(float2)));
         GGPU_HIP_TRY(hipDeviceSynchronize());

@@ -32,6 +32,7 @@ struct PrTailDev {
     const uint2* items;     // {first piece, end piece}
     uint32_t n_items;
     float2* part;           // [T][R] tile partials (zero where a row has no arc in a tile), then overflow slots
+    uint64_t n_part;        // entries of part
     const uint32_t* srow;   // split row -> vertex
     const uint32_t* slr;    // split row -> long-row index of its head chunks in the stream layout, or kNoHead
     const uint32_t* ovptr;  // split row -> overflow slots [ovptr[j], ovptr[j + 1])
@@ -86,7 +87,10 @@ __device__ __forceinline__ void pr_tail_body(const PrSweep& P, const PrTailDev& 
         }
         float hi = k.s, lo = -k.c;
         comp_reduce(hi, lo, G);
-        if (ok && gl == 0) Z.part[Z.pdst[q]] = make_float2(hi, lo);
+        if (ok && gl == 0) {
+            const uint32_t dst = Z.pdst[q];
+            if (dst < Z.n_part) Z.part[dst] = make_float2(hi, lo);  // always, with a consistent plan
+        }
     }
 }
 
@@ -131,7 +135,7 @@ __device__ __forceinline__ void pr_tail_merge_body(const PrSweep& P, const PrTai
             tail_add(H, L, x);
         }
         comp_reduce(H, L, kWave);
-        if (lane == 0) pr_store(P, Z.srow[j], add + d * (H + L), part, inv_n);
+        if (lane == 0 && Z.srow[j] < P.n) pr_store(P, Z.srow[j], add + d * (H + L), part, inv_n);
     }
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < Z.R; j += gridDim.x * blockDim.x) {
         const uint2 h = tail_head(P, Z, j);
@@ -141,7 +145,7 @@ __device__ __forceinline__ void pr_tail_merge_body(const PrSweep& P, const PrTai
         for (uint32_t q = 0; q < h.y; ++q) tail_add(H, L, P.stream.lpart[h.x + q]);
         for (uint32_t t = 0; t < Z.T; ++t) tail_add(H, L, Z.part[(size_t)t * Z.R + j]);
         for (uint32_t o = o0; o < o1; ++o) tail_add(H, L, Z.part[(size_t)Z.T * Z.R + o]);
-        pr_store(P, Z.srow[j], add + d * (H + L), part, inv_n);
+        if (Z.srow[j] < P.n) pr_store(P, Z.srow[j], add + d * (H + L), part, inv_n);
     }
     write_block_partials(part, P.partial, P.partial_cap, P.partial_base + blockIdx.x);
 }

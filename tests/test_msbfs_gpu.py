@@ -7,6 +7,8 @@ The batched BFS has no counterpart in the reference; its oracle is the
 reference's serial BFS (src/cpu/bfs_cpu.cpp:10-29, restated in
 oracle/oracle.c), run once per source.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -68,6 +70,36 @@ def test_batched_matches_serial_per_source(name, k, order):
     src = _batch(g, k)
     for mode, kw in MODES.items():
         _check(g, bfs, src, G.bfs_options(batch_order=ORDERS[order], **kw), f"{name} k={k} {mode} {order}")
+
+
+def _route(bfs, min_edges):
+    """Sets the routed top-down's minimum frontier arcs (0: the default); returns
+    the routed levels of the last run."""
+    last = ctypes.c_uint32()
+    bfs.ctx.check(bfs.ctx.lib.ggpu_test_msbfs_route(bfs._h, min_edges, ctypes.byref(last)))
+    return last.value
+
+
+@pytest.mark.parametrize("order", list(ORDERS))
+@pytest.mark.parametrize("name", SUITE)
+@pytest.mark.parametrize("k", [1, 64, 100, 256])
+def test_batched_routed_top_down(name, k, order):
+    """Every top-down level whose frontier has at most 8192 vertices routed
+    through the per-range LDS bins (queue, binned records, apply and settle),
+    under the heuristic and all top-down: every source's depths bit-exact."""
+    g, bfs = _graph(name)
+    src = _batch(g, k)
+    routed = 0
+    try:
+        _route(bfs, 1)
+        for mode in ("heuristic", "all-top-down"):
+            _check(g, bfs, src, G.bfs_options(batch_order=ORDERS[order], **MODES[mode]),
+                   f"{name} k={k} {mode} {order} routed")
+            routed += _route(bfs, 1)
+    finally:
+        _route(bfs, 0)
+    if g.m:
+        assert routed > 0, f"{name}: no level was routed"
 
 
 @pytest.mark.parametrize("name", ["rmat12", "grid100x100", "path1000", "star100k"])
