@@ -1275,7 +1275,10 @@ __global__ void __launch_bounds__(kMsThreads) k_msbfs(MsDev D) {
             // memory operations than one contended bitmap atomic per push)
             const bool scan_all = (uint64_t)t.edges * 8 > (uint64_t)D.n;
             const bool binned = act == GGPU_ACTION_TD_BINNED;
-            if (D.n_bins && t.size <= kRouteQ && t.edges >= D.route_min) {
+            // (routed only with one or two words per vertex: wider batches have bigger
+            // first frontiers than the queue holds, and the branch would cost their
+            // level loop registers)
+            if (W <= 2 && D.n_bins && t.size <= kRouteQ && t.edges >= D.route_min) {
                 // a small frontier with many arcs: routed through LDS bins
                 if (blockIdx.x == 0 && threadIdx.x == 0) D.rctl[1 + kRouteBins]++;  // routed levels of this run
                 ms_route_queue<W>(D, level);

@@ -29,10 +29,10 @@ int ggpu_test_scan(ggpu_ctx* ctx, uint32_t n, const uint32_t* in, uint32_t* out,
 int ggpu_test_frontier_convert(ggpu_bfs* b, const uint32_t* dist, uint32_t level, uint32_t tg, uint64_t* bitmap,
                                uint32_t* queue, uint32_t* degrees, uint32_t* count);
 
-/* Batched BFS, routed top-down: a top-down level whose frontier has at most
- * 8192 vertices and at least min_edges arcs is routed through per-range bins
- * in LDS (0: the default, 65536 arcs); *routed_last (may be null) = routed
- * levels of b's last batched run. */
+/* Batched BFS, routed top-down: a top-down level of a batch of at most 128
+ * sources whose frontier has at most 4096 vertices and at least min_edges arcs
+ * is routed through per-range bins in LDS (0: the default, 65536 arcs);
+ * *routed_last (may be null) = routed levels of b's last batched run. */
 int ggpu_test_msbfs_route(ggpu_bfs* b, uint64_t min_edges, uint32_t* routed_last);
 
 /* Sets the HOT_TAIL geometry of p's later layouts: the first tail source and
