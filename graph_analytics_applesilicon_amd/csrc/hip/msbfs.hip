@@ -178,13 +178,30 @@ __device__ __forceinline__ uint32_t ms_word_of(uint64_t r0, uint32_t lane, uint3
     return (uint32_t)(r0 + (uint64_t)lane * nw + gwave());
 }
 
+// Sums the kPartWords counters over the workgroup together (one LDS exchange
+// and one pair of barriers for all four), into every thread.
+__device__ __forceinline__ void ms_block_sum4(unsigned long long (&v)[kPartWords]) {
+    __shared__ unsigned long long s_red4[kPartWords][kMsThreads / kWave];
+    const uint32_t w = threadIdx.x / kWave, nwv = blockDim.x / kWave;
+#pragma unroll
+    for (uint32_t j = 0; j < kPartWords; ++j) v[j] = wave_sum(v[j]);
+    __syncthreads();  // the previous call's readers are done with s_red4
+    if (lane_id() == 0)
+#pragma unroll
+        for (uint32_t j = 0; j < kPartWords; ++j) s_red4[j][w] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kPartWords; ++j) {
+        unsigned long long t = 0;
+        for (uint32_t i = 0; i < nwv; ++i) t += s_red4[j][i];
+        v[j] = t;
+    }
+}
+
 // Per-block counters of the level being produced, one write-through store each.
 __device__ void ms_flush(const MsDev& D, uint32_t slot, const MsAcc& a) {
-    __shared__ unsigned long long s_red[kMsThreads / kWave];
-    const unsigned long long v[kPartWords] = {a.size, a.edges, a.pairs, a.examined};
-    unsigned long long t[kPartWords];
-#pragma unroll
-    for (uint32_t j = 0; j < kPartWords; ++j) t[j] = block_sum(v[j], s_red);
+    unsigned long long t[kPartWords] = {a.size, a.edges, a.pairs, a.examined};
+    ms_block_sum4(t);
     if (threadIdx.x == 0) {
         unsigned long long* p = D.part + ((size_t)slot * gridDim.x + blockIdx.x) * kPartWords;
 #pragma unroll
@@ -194,18 +211,18 @@ __device__ void ms_flush(const MsDev& D, uint32_t slot, const MsAcc& a) {
 
 // Totals of one counter slot, in every thread.
 __device__ MsAcc ms_totals(const MsDev& D, uint32_t slot) {
-    __shared__ unsigned long long s_red[kMsThreads / kWave];
     unsigned long long v[kPartWords] = {0, 0, 0, 0};
     for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
         const unsigned long long* p = D.part + ((size_t)slot * gridDim.x + i) * kPartWords;
 #pragma unroll
         for (uint32_t j = 0; j < kPartWords; ++j) v[j] += p[j];
     }
+    ms_block_sum4(v);
     MsAcc a;
-    a.size = block_sum(v[0], s_red);
-    a.edges = block_sum(v[1], s_red);
-    a.pairs = block_sum(v[2], s_red);
-    a.examined = block_sum(v[3], s_red);
+    a.size = v[0];
+    a.edges = v[1];
+    a.pairs = v[2];
+    a.examined = v[3];
     return a;
 }
 
