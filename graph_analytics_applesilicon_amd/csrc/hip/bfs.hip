@@ -176,6 +176,7 @@ BfsEngine::BfsEngine(ggpu_ctx* c, uint64_t n_, uint64_t m_, const uint32_t* hrp,
     stats.alloc(kStatSlots * kSlotWords + kConvWords);
     GGPU_HIP_TRY(hipMemcpyAsync(rp.p, hrp, 4ull * (n + 1), hipMemcpyHostToDevice, ctx->stream));
     if (m) GGPU_HIP_TRY(hipMemcpyAsync(ci.p, hci, 4ull * m, hipMemcpyHostToDevice, ctx->stream));
+    else GGPU_HIP_TRY(hipMemsetAsync(ci.p, 0, 4, ctx->stream));  // the placeholder arc: a valid vertex id
     ensure_edge_buffers();
     ensure_bitmaps();
     // Bottom-up heavy items: vertices of degree >= kHeavyMin, lists longer than

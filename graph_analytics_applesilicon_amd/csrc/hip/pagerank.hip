@@ -217,7 +217,9 @@ template <class T, class H>
 static void upload_vec(DBuf<T>& d, const std::vector<H>& h) {
     static_assert(sizeof(T) == sizeof(H), "host and device records differ in size");
     d.alloc(std::max<size_t>(h.size(), 1));
+    // an empty list keeps one zeroed placeholder (clamped loads read element 0 and use it as an index)
     if (!h.empty()) GGPU_HIP_TRY(hipMemcpy(d.p, h.data(), h.size() * sizeof(H), hipMemcpyHostToDevice));
+    else GGPU_HIP_TRY(hipMemset(d.p, 0, sizeof(T)));
 }
 
 // One vertex numbering of the in-CSR with its device copy and work layouts.
@@ -307,7 +309,10 @@ struct PrEngine {
         S.ci.alloc(S.in.m ? S.in.m : 1);
         S.odd.alloc(n ? n : 1);
         GGPU_HIP_TRY(hipMemcpy(S.rp.p, S.in.rp.data(), 4ull * (n + 1), hipMemcpyHostToDevice));
+        // a graph without arcs keeps one placeholder arc, source 0: the TILED sweep's
+        // clamped loads read arc 0 for empty rows and gather the contribution it names
         if (S.in.m) GGPU_HIP_TRY(hipMemcpy(S.ci.p, S.in.ci.data(), 4ull * S.in.m, hipMemcpyHostToDevice));
+        else GGPU_HIP_TRY(hipMemset(S.ci.p, 0, 4));
         if (n) GGPU_HIP_TRY(hipMemcpy(S.odd.p, S.od.data(), 4ull * n, hipMemcpyHostToDevice));
         if (S.ranges.empty()) S.ranges.push_back({0u, n});
         S.built = true;
@@ -337,6 +342,7 @@ struct PrEngine {
         S.oci.alloc(out.m ? out.m : 1);
         GGPU_HIP_TRY(hipMemcpy(S.orp.p, out.rp.data(), 4ull * (n + 1), hipMemcpyHostToDevice));
         if (out.m) GGPU_HIP_TRY(hipMemcpy(S.oci.p, out.ci.data(), 4ull * out.m, hipMemcpyHostToDevice));
+        else GGPU_HIP_TRY(hipMemset(S.oci.p, 0, 4));
         std::vector<uint32_t> heavy;
         for (uint32_t v = 0; v < n; ++v)
             if (out.deg(v) >= kPushHeavy) heavy.push_back(v);

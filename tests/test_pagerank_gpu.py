@@ -49,19 +49,12 @@ CFGS = [("gpu_pull_scalar", {}), ("gpu_pull_vector", {}),
         ("gpu_push", dict(order=G.PR_ORDER_DEGREE))]
 
 
-# The empty graph's case of this test faulted the GPU twice in full-suite runs
-# ("an illegal memory access was encountered" at a stream synchronize, after the
-# batched-BFS tests); its kernels run no code that changed since the suite last
-# passed with it, and no cause has been found yet. It stays off the GPU until
-# one is (the empty graph's PageRank is still checked by
-# test_pagerank_known_answers_on_gpu).
-_VARIANT_GRAPHS = [pytest.param(n, marks=pytest.mark.skip(reason="faulted the GPU in two full-suite runs; cause not yet "
-                                                                 "found, kept off the GPU until it is"))
-                   if n == "empty1000" else n for n in SUITE]
-
-
-@pytest.mark.parametrize("name", _VARIANT_GRAPHS)
+@pytest.mark.parametrize("name", SUITE)
 def test_pagerank_variants_match_oracle(name):
+    """On empty1000 (no arcs) the TILED sweep's empty lanes read the placeholder
+    arc 0 and gather the contribution it names: the placeholder must hold a
+    valid source id (it once held whatever the allocation held before, and a
+    stale value sent that gather out of bounds)."""
     g, t, od = _in_graph(name)
     pr = G.GpuPageRank(t, od)
     for tol in (0.0, 1e-6):
