@@ -16,7 +16,10 @@
 //     vertex keeps next & ~visit as its new bits, so each (source, vertex)
 //     pair is discovered once.
 // Vertices of degree >= heavy_min are scanned / pushed as chunk-edge items
-// by whole waves (both directions), the rest lane per vertex.
+// by whole waves (both directions), the rest lane per vertex. A top-down
+// level with a small frontier and many arcs is routed instead of pushed
+// (batches of up to 128 sources): its records are binned by target range and
+// each bin's pushes are applied with LDS atomics (ms_route_*, below).
 // Depths are not written during the traversal. A launch of k_msbfs ends when
 // its C level slots are used up (saving the loop state; the next launch
 // resumes there) or when the traversal is over; then k_ms_finish turns the
