@@ -828,52 +828,89 @@ __device__ void ms_route_apply(const MsDev& D, uint32_t level, uint32_t S, MsAcc
             for (uint32_t x = t; x < (hi - lo) * W; x += T) sNext[x] = 0;
             __syncthreads();
             const uint32_t* rec = D.recs + D.bbase[b];
-            for (uint32_t i = t; i < cnt; i += T) {
-                // written through before the barrier, whose acquire dropped stale lines: plain loads
-                const uint32_t r = rec[i];
-                const uint32_t off = r >> kRouteQBits, q = r & (kRouteQ - 1);
-                if (off >= hi - lo || q >= S) continue;  // not a record of this level (never with matching counters)
+            // kApply records per thread per round: their loads, then their frontier
+            // words, then the LDS atomics (two memory round trips per round)
+            constexpr uint32_t kApply = W == 1 ? 8 : 4;
+            for (uint32_t base = 0; base < cnt; base += kApply * T) {
+                uint32_t r[kApply];
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    const uint64_t f = D.rqf[(size_t)q * W + w];
-                    if (f) atomicOr((unsigned long long*)&sNext[(size_t)off * W + w], (unsigned long long)f);
+                for (uint32_t j = 0; j < kApply; ++j) {
+                    // written through before the barrier, whose acquire dropped stale lines: plain loads
+                    const uint32_t i = base + j * T + t;
+                    r[j] = i < cnt ? rec[i] : 0xFFFFFFFFu;
+                }
+                uint64_t f[kApply][W];
+#pragma unroll
+                for (uint32_t j = 0; j < kApply; ++j) {
+                    const uint32_t off = r[j] >> kRouteQBits, q = r[j] & (kRouteQ - 1);
+                    // past the count, or not a record of this level (never with matching counters): nothing to OR
+                    const bool ok = off < hi - lo && q < S;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) f[j][w] = ok ? D.rqf[(size_t)q * W + w] : 0ull;
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kApply; ++j) {
+                    const uint32_t off = r[j] >> kRouteQBits;
+#pragma unroll
+                    for (int w = 0; w < W; ++w)
+                        if (f[j][w]) atomicOr((unsigned long long*)&sNext[(size_t)off * W + w], (unsigned long long)f[j][w]);
                 }
             }
             __syncthreads();
-            // settle [lo, hi): a wave per 64-vertex word, as ms_td_collect_all
-            for (uint32_t u0 = lo + wv * kWave; u0 < hi; u0 += nwv * kWave) {
-                const uint32_t u = u0 + lane;
-                uint32_t pc = 0;
-                bool full = u < D.n;
-                if (u < D.n) {
+            // settle [lo, hi) as ms_td_collect_all does, a wave taking kSettle 64-vertex
+            // words per round: the round's visit words and row offsets are loaded for
+            // every vertex of the bin (unconditional loads: made conditional on the
+            // vertex's next words, they took half again as long on a 16384-vertex bin)
+            // before any is used
+            constexpr uint32_t kSettle = W == 1 ? 4 : 2;
+            for (uint32_t u00 = lo + wv * kWave * kSettle; u00 < hi; u00 += nwv * kWave * kSettle) {
+                uint64_t nx[kSettle][W], vis[kSettle][W];
+                uint32_t r0[kSettle], r1[kSettle];
+#pragma unroll
+                for (uint32_t j = 0; j < kSettle; ++j) {
+                    const uint32_t u = u00 + j * kWave + lane;
+                    const bool in = u < hi && u < D.n;
+                    const uint32_t uc = in ? u : 0u;
 #pragma unroll
                     for (int w = 0; w < W; ++w) {
-                        const size_t i = (size_t)u * W + w;
-                        const uint64_t nx = sNext[(size_t)(u - lo) * W + w];
-                        full &= nx != 0;
-                        if (nx) {
-                            const uint64_t vis = D.visit[i];
-                            const uint64_t nwb = nx & ~vis;
-                            full &= (vis | nwb) == valid_word<W>(D, w);
+                        nx[j][w] = in ? sNext[(size_t)(u - lo) * W + w] : 0ull;
+                        vis[j][w] = D.visit[(size_t)uc * W + w];
+                    }
+                    r0[j] = D.rp[uc];
+                    r1[j] = D.rp[uc + 1];
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kSettle; ++j) {
+                    const uint32_t u0 = u00 + j * kWave, u = u0 + lane;
+                    if (u0 >= hi) break;  // wave-uniform
+                    uint32_t pc = 0;
+                    bool full = u < D.n;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        full &= nx[j][w] != 0;
+                        if (nx[j][w]) {
+                            const size_t i = (size_t)u * W + w;
+                            const uint64_t nwb = nx[j][w] & ~vis[j][w];
+                            full &= (vis[j][w] | nwb) == valid_word<W>(D, w);
                             if (nwb) {
                                 store_through(&Fn[i], nwb);
-                                store_through(&D.visit[i], vis | nwb);
+                                store_through(&D.visit[i], vis[j][w] | nwb);
                                 pc += (uint32_t)__popcll(nwb);
                             }
                         }
                     }
-                }
-                const uint64_t found = __ballot(pc != 0);
-                const uint64_t cmask = __ballot(pc != 0 && full);
-                if (lane == 0 && cmask) atomicOr((unsigned long long*)&D.done[u0 >> 6], (unsigned long long)cmask);
-                if (pc) {
-                    const uint64_t dg = D.rp[u + 1] - D.rp[u];
-                    a.pairs += dg * pc;
-                    a.edges += dg;
-                }
-                if (lane == 0 && found) {
-                    store_through(&fbn[u0 >> 6], found);
-                    a.size += (uint64_t)__popcll(found);
+                    const uint64_t found = __ballot(pc != 0);
+                    const uint64_t cmask = __ballot(pc != 0 && full);
+                    if (lane == 0 && cmask) atomicOr((unsigned long long*)&D.done[u0 >> 6], (unsigned long long)cmask);
+                    if (pc) {
+                        const uint64_t dg = r1[j] - r0[j];
+                        a.pairs += dg * pc;
+                        a.edges += dg;
+                    }
+                    if (lane == 0 && found) {
+                        store_through(&fbn[u0 >> 6], found);
+                        a.size += (uint64_t)__popcll(found);
+                    }
                 }
             }
             if (t == 0) store_through(&D.rctl[1 + b], 0u);  // the bin is empty for the next routed level
