@@ -558,7 +558,7 @@ def pagerank_part(G, O, g, ctx, g_name="rmat23_c5", with_ref=True):
     per_sweep_ms = sweeps[best_name]
     ach = bytes_sweep / (per_sweep_ms / 1e3) / 1e9
     kname = {G.PR_LAYOUT_HOT: "k_pr_sweep_hot", G.PR_LAYOUT_TILED: "k_pr_sweep_tiled",
-             G.PR_LAYOUT_HOT_TAIL: "k_pr_sweep_hot + k_pr_tail + k_pr_tail_merge"}.get(layout, "k_pr_sweep")
+             G.PR_LAYOUT_HOT_TAIL: "k_pr_sweep_hot + k_pr_band + k_pr_tail + k_pr_tail_merge"}.get(layout, "k_pr_sweep")
     res["roofline"] = {"kernel": f"{kname} (gpu_pull_adaptive, Jacobi, {best_name} layout/order)",
                        "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(kname, g_name),

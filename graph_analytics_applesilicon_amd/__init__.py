@@ -195,7 +195,7 @@ def _optional_hip_symbols():
         "ggpu_test_frontier_convert": ([vp, _u32p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64), _u32p, _u32p,
                                         C.POINTER(C.c_uint32)], C.c_int),
         "ggpu_test_msbfs_route": ([vp, C.c_uint64, C.POINTER(C.c_uint32)], C.c_int),
-        "ggpu_test_pr_tail_geometry": ([vp, C.c_uint32, C.c_uint32], C.c_int),
+        "ggpu_test_pr_tail_geometry": ([vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32], C.c_int),
         "ggpu_test_pr_tail_info": ([vp, C.c_uint32, C.POINTER(C.c_uint64)], C.c_int),
         "ggpu_bfs_run_noalloc": ([vp, C.c_uint64, C.c_uint64, vp, vp, C.c_uint32, C.POINTER(BfsOpts), vp,
                                   C.POINTER(C.c_uint32)], C.c_int),

@@ -38,10 +38,12 @@ constexpr uint32_t kSecBlocks = 2048;             // block cap per section (grid
 constexpr uint32_t kStreamBlocks = 8192;          // block cap of a stream-layout sweep
 constexpr uint32_t kPartialCap = 1u << 17;        // FP64 partial slots per quantity (all launches of a sweep)
 constexpr uint32_t kPushHeavy = 256;              // gpu_push: out-degree from which a wave pushes a source
-// HOT_TAIL geometry: sources below kTailBegin are the LDS copy (30720) and a
-// ~2 MB band that stays in L2; tail tiles of kTailTile sources are 2 MB of
-// contributions each (one XCD's L2 is 4 MB).
-constexpr uint32_t kTailBegin = 1u << 19;
+// HOT_TAIL geometry: the split rows' heads are the sources of the HOT sweep's
+// LDS copy (below kHotCap); kBandTiles band tiles of kBandTile sources follow
+// (64 KiB of contributions each, copied to LDS by k_pr_band), then tail tiles
+// of kTailTile sources (2 MiB of contributions each; one XCD's L2 is 4 MiB).
+constexpr uint32_t kTailBegin = kHotCap;
+constexpr uint32_t kBandTiles = 30;
 constexpr uint32_t kTailTile = 1u << 19;
 constexpr uint32_t kTailMergeBlocks = 16384;       // block cap of the tail merge (grid-stride beyond)
 
@@ -52,7 +54,14 @@ __global__ void __launch_bounds__(kHotThreads) k_pr_sweep_hot(PrSweep P) { pr_ho
 __global__ void k_pr_longfix(PrSweep P) { pr_longfix_body(P); }
 __global__ void __launch_bounds__(kTileThreads) k_pr_sweep_tiled(PrSweep P, PrTiledDev Z) { pr_tiled_body(P, Z); }
 __global__ void __launch_bounds__(256) k_pr_tiled_merge(PrSweep P, PrTiledDev Z) { pr_tiled_merge_body(P, Z); }
-__global__ void __launch_bounds__(kTailWaves * kWave) k_pr_tail(PrSweep P, PrTailDev Z) { pr_tail_body(P, Z); }
+__global__ void __launch_bounds__(kTailWaves * kWave) k_pr_tail(PrSweep P, PrTailDev Z, uint32_t item_base) {
+    pr_tail_body(P, Z, item_base);
+}
+// two workgroups per CU (their LDS tiles and registers allow it)
+__global__ void __launch_bounds__(kBandWaves * kWave, 2)
+    k_pr_band(PrSweep P, PrTailDev Z, const uint4* tasks, uint32_t tail_begin, uint32_t band_tile) {
+    pr_band_body(P, Z, tasks, tail_begin, band_tile);
+}
 __global__ void __launch_bounds__(256) k_pr_tail_merge(PrSweep P, PrTailDev Z) { pr_tail_merge_body(P, Z); }
 __global__ void k_pr_reduce(const double* partial, uint32_t cap, uint32_t count, PrAcc* out) {
     pr_reduce_body(partial, cap, count, out);
@@ -184,12 +193,14 @@ struct TiledLayout {
 
 // The HOT_TAIL layout of one numbering: the HOT items with the split rows'
 // heads as flagged long rows (stream), the tail plan (pr_tail_plan.hpp) on the
-// device, and the partial array.
+// device, the band tasks, and the partial array.
 struct TailLayout {
-    uint32_t min_indeg = 0, tail_begin = 0, tile = 0;
+    uint32_t min_indeg = 0, tail_begin = 0, tile = 0, band_end = 0, band_tile = 0;
     StreamLayout stream;
     uint32_t R = 0, T = 0, n_items = 0, n_big = 0;
+    uint32_t band_tiles = 0, n_tasks = 0, tail_item0 = 0;  // items [0, tail_item0) are the band tiles'
     uint64_t tail_arcs = 0, n_part = 0;
+    DBuf<uint4> tasks;  // band tasks {tile, first item, end item, 0}
     DBuf<uint32_t> tci, px, pdst, srow, slr, ovptr, big;
     DBuf<uint2> items;
     DBuf<float2> part;
@@ -260,8 +271,8 @@ struct PrEngine {
     DBuf<double> partial;
     uint32_t npart = 0;      // partial slots written since the last reduction
     DBuf<PrAcc> acc;         // [0],[1] Jacobi ping-pong / in-place state, [2] in-place sweep totals
-    // HOT_TAIL geometry: first tail source and sources per tile (test hook overrides; 0: the defaults)
-    uint32_t tail_begin_set = 0, tail_tile_set = 0;
+    // HOT_TAIL geometry (test hook overrides; tail_begin and tile both 0: the defaults)
+    uint32_t tail_begin_set = 0, tail_tile_set = 0, band_end_set = 0, band_tile_set = 0;
     // Gauss-Seidel decision (made on the caller's numbering)
     bool gs_ready = false, gs_colored = false;
     uint32_t gs_pair_k = 0;
@@ -493,20 +504,29 @@ struct PrEngine {
     // head [rp[v], split) becomes long-row chunks flagged kLrowSplit (their
     // partials are always stored), and the tail plan with its partial array.
     void build_tail(Space& S, uint32_t min_indeg) {
-        const uint32_t tb = tail_begin_set ? tail_begin_set : kTailBegin;
-        const uint32_t tile = tail_tile_set ? tail_tile_set : kTailTile;
-        if (S.tail && S.tail->min_indeg == min_indeg && S.tail->tail_begin == tb && S.tail->tile == tile) return;
+        const bool dflt = !tail_begin_set && !tail_tile_set;
+        const uint32_t tb = dflt ? kTailBegin : tail_begin_set;
+        const uint32_t tile = dflt ? kTailTile : tail_tile_set;
+        const uint32_t bt = dflt ? kBandTile : band_tile_set;
+        const uint32_t be = dflt ? kTailBegin + kBandTiles * kBandTile : (bt ? band_end_set : 0u);
+        if (S.tail && S.tail->min_indeg == min_indeg && S.tail->tail_begin == tb && S.tail->tile == tile &&
+            S.tail->band_end == be && S.tail->band_tile == bt)
+            return;
         TailParams prm;
         prm.min_indeg = min_indeg;
         prm.tail_begin = tb;
         prm.tile = tile;
         prm.nnz_cap = kWaveNnz;
         prm.pieces_cap = kWaveNnz;
+        prm.band_end = be;
+        prm.band_tile = bt;
         const TailPlan plan = plan_tail(S.in, prm);
         auto T = std::make_unique<TailLayout>();
         T->min_indeg = min_indeg;
         T->tail_begin = tb;
         T->tile = tile;
+        T->band_end = be;
+        T->band_tile = bt;
         T->R = plan.R;
         T->T = plan.T;
         T->n_items = (uint32_t)plan.items.size();
@@ -569,6 +589,24 @@ struct PrEngine {
             if (heads + (plan.ovptr[r + 1] - plan.ovptr[r]) > kMergeLanePartials) big.push_back(r);
         }
         T->n_big = (uint32_t)big.size();
+        // band tasks: runs of <= kBandTaskItems consecutive items of one band tile
+        // (items are tile-major, so the band tiles' items come first)
+        std::vector<uint4> tasks;
+        if (plan.R) {
+            T->band_tiles = tail_band_tiles(prm, n);
+            auto tile_of_item = [&](uint32_t i) { return tail_tile_of(prm, plan.tci[plan.px[plan.items[i][0]]]); };
+            uint32_t i = 0;
+            while (i < T->n_items && tile_of_item(i) < T->band_tiles) {
+                const uint32_t t = tile_of_item(i);
+                uint32_t e = i + 1;
+                while (e < T->n_items && e - i < kBandTaskItems && tile_of_item(e) == t) ++e;
+                tasks.push_back(make_uint4(t, i, e, 0));
+                i = e;
+            }
+            T->tail_item0 = i;
+        }
+        T->n_tasks = (uint32_t)tasks.size();
+        upload_vec(T->tasks, tasks);
         upload_vec(T->tci, plan.tci);
         upload_vec(T->px, plan.px);
         upload_vec(T->pdst, plan.pdst);
@@ -786,9 +824,14 @@ struct PrEngine {
             launch(P, use_stream(P, T.stream, 2), o.tg);
             if (T.R) {
                 const PrTailDev Z = T.dev();
-                if (T.n_items) {
-                    const uint32_t g = (T.n_items + kTailWaves - 1) / kTailWaves;  // one item per wave
-                    k_pr_tail<<<g, kTailWaves * kWave, 0, ctx->stream>>>(P, Z);
+                if (T.n_tasks) {
+                    k_pr_band<<<T.n_tasks, kBandWaves * kWave, 0, ctx->stream>>>(P, Z, T.tasks.p, T.tail_begin,
+                                                                                 T.band_tile);
+                    GGPU_HIP_TRY(hipGetLastError());
+                }
+                if (T.n_items > T.tail_item0) {
+                    const uint32_t g = (T.n_items - T.tail_item0 + kTailWaves - 1) / kTailWaves;  // one item per wave
+                    k_pr_tail<<<g, kTailWaves * kWave, 0, ctx->stream>>>(P, Z, T.tail_item0);
                     GGPU_HIP_TRY(hipGetLastError());
                 }
                 // a lane per split row, and a wave per row of the `big` list (no wave
@@ -1141,17 +1184,22 @@ int ggpu_ppr_fetch(ggpu_pr* p, float* out) {
     return guarded(p->ctx, [&] { ppr_fetch(*p->eng, out); });
 }
 
-int ggpu_test_pr_tail_geometry(ggpu_pr* p, uint32_t tail_begin, uint32_t tile) {
+int ggpu_test_pr_tail_geometry(ggpu_pr* p, uint32_t tail_begin, uint32_t tile, uint32_t band_end,
+                               uint32_t band_tile) {
     if (!p) return GGPU_EINVAL;
+    const bool dflt = !tail_begin && !tile;
+    if (!dflt && (!tile || band_tile > kBandTile || (band_tile && band_end <= tail_begin))) return GGPU_EINVAL;
     p->eng->tail_begin_set = tail_begin;
     p->eng->tail_tile_set = tile;
+    p->eng->band_end_set = band_end;
+    p->eng->band_tile_set = band_tile;
     return GGPU_OK;
 }
 
 int ggpu_test_pr_tail_info(const ggpu_pr* p, uint32_t order, uint64_t* stats) {
     if (!p || !stats || order > GGPU_PR_ORDER_DEGREE) return GGPU_EINVAL;
     const Space& S = p->eng->sp[order == GGPU_PR_ORDER_DEGREE ? SP_DEGREE : SP_GIVEN];
-    for (int i = 0; i < 6; ++i) stats[i] = 0;
+    for (int i = 0; i < 8; ++i) stats[i] = 0;
     if (!S.tail) return GGPU_OK;
     const TailLayout& T = *S.tail;
     stats[0] = T.R;
@@ -1160,6 +1208,8 @@ int ggpu_test_pr_tail_info(const ggpu_pr* p, uint32_t order, uint64_t* stats) {
     stats[3] = T.n_items;
     stats[4] = T.n_big;
     stats[5] = T.stream.nl;
+    stats[6] = T.band_tiles;
+    stats[7] = T.n_tasks;
     return GGPU_OK;
 }
 

@@ -2,17 +2,24 @@
 // No reference counterpart: the reference pulls every row from the whole
 // contribution vector (src/shaders/pagerank.metal:306-381).
 //
-// In the degree-ordered numbering the tail sources [tail_begin, n) hold a few
-// per cent of the gathers but, spread over most of the contribution vector,
-// every one of them misses L2. The split rows' tail arcs are copied tile-major
-// (tile t: sources [tail_begin + t * tile, + tile), a tile of contributions
-// small enough for one XCD's L2). k_pr_tail runs one wave per item in item
-// order (tile-major), so the waves in flight read one or two tiles, which stay
-// in L2: each wave stages its item's gathers in LDS and sums each piece (<= 512
-// arcs of one split row in one tile) with Kahan + TwoSum into a (hi, lo)
-// partial. The HOT sweep leaves the split rows' heads (sources below
-// tail_begin) as chunk partials; k_pr_tail_merge adds, per split row, its head
-// partials in chunk order, then its tile partials in tile order, then its
+// In the degree-ordered numbering the HOT sweep serves sources [0, 30720) from
+// LDS and gathers the rest through L2. The split rows (the heavy ones) give
+// most of those gathers; their arcs from sources >= tail_begin are copied
+// tile-major and summed per (tile, row) piece (<= 512 arcs of one split row in
+// one tile) with Kahan + TwoSum into a (hi, lo) partial, by two kernels:
+//  * k_pr_band, for the band tiles (band_tile <= kBandTile sources each): a
+//    workgroup per task of <= kBandTaskItems items of one tile copies the
+//    tile's contributions to LDS, and G lanes per piece sum the piece's arcs
+//    from there. 64 KiB of LDS and few registers leave room for two 1024-lane
+//    workgroups per CU: the waves are chains of dependent loads (item, piece
+//    bounds, arc ids), and twice the waves in flight is what made the band pay;
+//  * k_pr_tail, for the tail tiles beyond the band (tiles small enough for
+//    one XCD's L2): one wave per item in item order (tile-major), so the waves
+//    in flight read one or two tiles, which stay in L2; each wave stages its
+//    item's gathers in LDS, then sums each piece.
+// The HOT sweep leaves the split rows' heads (sources below tail_begin) as
+// chunk partials; k_pr_tail_merge adds, per split row, its head partials in
+// chunk order, then its tile partials in tile order (band tiles first), then its
 // overflow partials, with TwoSum, and stores the row (pr_store: ranks,
 // contributions and the FP64 dangling / sum / change partials).
 #pragma once
@@ -22,6 +29,9 @@
 namespace ggpu {
 
 constexpr uint32_t kTailWaves = 4;             // waves per k_pr_tail workgroup (one item each)
+constexpr uint32_t kBandTile = 16384;          // most sources per band tile: 64 KiB of LDS
+constexpr uint32_t kBandWaves = 16;            // waves per k_pr_band workgroup
+constexpr uint32_t kBandTaskItems = 64;        // items per band task
 constexpr uint32_t kNoHead = 0xFFFFFFFFu;      // a split row whose in-list lies entirely in the tail
 constexpr uint32_t kMergeLanePartials = 8;     // split rows with more head + overflow partials: a wave each
 
@@ -45,10 +55,11 @@ struct PrTailDev {
 // per lane, tci streamed with non-temporal loads) into the wave's LDS staging,
 // then G lanes per piece sum it (G: the largest power of two with G * pieces
 // <= 64).
-__device__ __forceinline__ void pr_tail_body(const PrSweep& P, const PrTailDev& Z) {
+// Items [item_base, n_items): the tail tiles' items.
+__device__ __forceinline__ void pr_tail_body(const PrSweep& P, const PrTailDev& Z, uint32_t item_base) {
     __shared__ float s_stage[kTailWaves][kWaveNnz];
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
-    const uint32_t it = blockIdx.x * kTailWaves + w;
+    const uint32_t it = item_base + blockIdx.x * kTailWaves + w;
     if (it >= Z.n_items) return;  // wave-uniform; no workgroup barrier follows
     const uint2 item = Z.items[it];
     const uint32_t x0 = Z.px[item.x], cnt = Z.px[item.y] - x0;
@@ -90,6 +101,55 @@ __device__ __forceinline__ void pr_tail_body(const PrSweep& P, const PrTailDev& 
         if (ok && gl == 0) {
             const uint32_t dst = Z.pdst[q];
             if (dst < Z.n_part) Z.part[dst] = make_float2(hi, lo);  // always, with a consistent plan
+        }
+    }
+}
+
+// One workgroup per band task {tile, first item, end item}: the tile's
+// contributions (sources tail_begin + tile * band_tile onwards) are copied to
+// LDS, then each wave takes the task's items w, w + kBandWaves, ... (the next
+// item's descriptor loaded while the current one is summed) and G lanes per
+// piece (G: the largest power of two with G * pieces <= 64) sum its arcs from
+// the LDS tile, eight arc ids in flight per lane, each lane in arc order.
+__device__ __forceinline__ void pr_band_body(const PrSweep& P, const PrTailDev& Z, const uint4* tasks,
+                                             uint32_t tail_begin, uint32_t band_tile) {
+    __shared__ float s_tile[kBandTile];
+    const uint4 task = tasks[blockIdx.x];
+    const uint32_t tb = tail_begin + task.x * band_tile;
+    const uint32_t tcnt = min(band_tile, P.n - tb);  // the host keeps band_tile <= kBandTile and tb < n
+    for (uint32_t i = threadIdx.x; i < tcnt; i += blockDim.x) s_tile[i] = P.contrib_in[tb + i];
+    __syncthreads();
+    const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+    uint2 item = task.y + w < task.z ? Z.items[task.y + w] : make_uint2(0, 0);
+    for (uint32_t it = task.y + w; it < task.z; it += kBandWaves) {
+        const uint2 cur = item;
+        item = it + kBandWaves < task.z ? Z.items[it + kBandWaves] : make_uint2(0, 0);
+        const uint32_t pieces = cur.y - cur.x;
+        uint32_t G = pieces >= kWave ? 1u : kWave / pieces;
+        G = G >= 64 ? 64 : (G >= 32 ? 32 : (G >= 16 ? 16 : (G >= 8 ? 8 : (G >= 4 ? 4 : (G >= 2 ? 2 : 1)))));
+        const uint32_t per_round = kWave / G, rr = lane / G, gl = lane % G;
+        for (uint32_t pb = 0; pb < pieces; pb += per_round) {
+            const uint32_t q = cur.x + pb + rr;
+            const bool ok = pb + rr < pieces;
+            const uint32_t a = ok ? Z.px[q] : 0u, b = ok ? Z.px[q + 1] : 0u;
+            Kahan k;
+            for (uint32_t x = a + gl; x < b; x += 8 * G) {
+                uint32_t u[8];
+#pragma unroll
+                for (uint32_t j = 0; j < 8; ++j) u[j] = Z.tci[min(x + j * G, b - 1)];
+#pragma unroll
+                for (uint32_t j = 0; j < 8; ++j) {
+                    // a consistent plan keeps u in the tile: the clamp only keeps LDS reads in bounds
+                    const float v = s_tile[min(u[j] - tb, kBandTile - 1)];
+                    if (x + j * G < b) k.add(v);
+                }
+            }
+            float hi = k.s, lo = -k.c;
+            comp_reduce(hi, lo, G);
+            if (ok && gl == 0) {
+                const uint32_t dst = Z.pdst[q];
+                if (dst < Z.n_part) Z.part[dst] = make_float2(hi, lo);  // always, with a consistent plan
+            }
         }
     }
 }

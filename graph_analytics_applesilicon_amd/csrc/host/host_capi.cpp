@@ -349,6 +349,8 @@ extern "C" int ggpu_test_pr_tail_emulate(uint64_t n, uint64_t m, const uint32_t*
         p.tile = params[2];
         p.nnz_cap = params[3];
         p.pieces_cap = params[4];
+        p.band_end = params[5];
+        p.band_tile = params[6];
         const ggpu::TailPlan plan = ggpu::plan_tail(in, p);
         ggpu::emulate_tail(in, p, plan, contrib, sums);
         stats[0] = plan.R;
@@ -358,6 +360,6 @@ extern "C" int ggpu_test_pr_tail_emulate(uint64_t n, uint64_t m, const uint32_t*
         stats[4] = plan.pdst.size();
         stats[5] = plan.items.size();
         stats[6] = plan.ovptr.back();
-        stats[7] = 0;
+        stats[7] = plan.T ? ggpu::tail_band_tiles(p, n) : 0;
     });
 }

@@ -21,13 +21,14 @@ constexpr uint32_t kNoSplit = 0xFFFFFFFFu;
 
 TailPlan plan_tail(const Csr& in, const TailParams& p) {
     if (p.tile == 0) fail("tile must be >= 1");
+    if (p.band_tile && p.band_end <= p.tail_begin) fail("a band must end after the first tail source");
     if (p.nnz_cap < 64 || p.nnz_cap % 64 || p.pieces_cap == 0) fail("bad item limits");
     TailPlan P;
     P.px.push_back(0);
     P.ovptr.push_back(0);
     const uint64_t n = in.n;
     if (n <= p.tail_begin) return P;  // no tail sources
-    const uint64_t T = (n - p.tail_begin + p.tile - 1) / p.tile;
+    const uint64_t T = tail_tiles(p, n);
 
     // split rows: in-degree >= min_indeg, a sorted in-list (so that its tail is a
     // suffix) and at least one tail arc
@@ -58,7 +59,7 @@ TailPlan plan_tail(const Csr& in, const TailParams& p) {
     for (int64_t j = 0; j < (int64_t)R; ++j) {
         const uint32_t* q = in.ci.data() + P.split[j];
         const uint32_t* e = in.nbr_end(P.srow[j]);
-        for (; q != e; ++q) len[(size_t)((*q - p.tail_begin) / p.tile) * R + j]++;
+        for (; q != e; ++q) len[(size_t)tail_tile_of(p, *q) * R + j]++;
     }
     // overflow slots: every piece after the first of a segment
     P.ovptr.assign(R + 1, 0);
@@ -87,9 +88,9 @@ TailPlan plan_tail(const Csr& in, const TailParams& p) {
         const uint32_t* q = in.ci.data() + P.split[j];
         const uint32_t* e = in.nbr_end(P.srow[j]);
         while (q != e) {
-            const uint64_t t = (*q - p.tail_begin) / p.tile;
+            const uint64_t t = tail_tile_of(p, *q);
             const uint32_t* r = q;
-            while (r != e && (*r - p.tail_begin) / p.tile == t) ++r;
+            while (r != e && tail_tile_of(p, *r) == t) ++r;
             std::copy(q, r, P.tci.begin() + off[t * R + j]);
             q = r;
         }
@@ -149,11 +150,11 @@ void emulate_tail(const Csr& in, const TailParams& p, const TailPlan& P, const f
         for (uint32_t q = it[0]; q < it[1]; ++q) {
             const uint32_t a = P.px[q], b = P.px[q + 1];
             if (b <= a || b - a > p.nnz_cap) fail("bad piece size");
-            const int64_t t = ((int64_t)P.tci[a] - p.tail_begin) / p.tile;
+            const int64_t t = P.tci[a] < p.tail_begin ? -1 : (int64_t)tail_tile_of(p, P.tci[a]);
             double s = 0.0;
             for (uint32_t x = a; x < b; ++x) {
                 const uint32_t u = P.tci[x];
-                if (u < p.tail_begin || u >= n || (int64_t)((u - p.tail_begin) / p.tile) != t)
+                if (u < p.tail_begin || u >= n || (int64_t)tail_tile_of(p, u) != t)
                     fail("a piece leaves its tile");
                 s += (double)contrib[u];
             }

@@ -35,15 +35,19 @@ int ggpu_test_frontier_convert(ggpu_bfs* b, const uint32_t* dist, uint32_t level
  * *routed_last (may be null) = routed levels of b's last batched run. */
 int ggpu_test_msbfs_route(ggpu_bfs* b, uint64_t min_edges, uint32_t* routed_last);
 
-/* Sets the HOT_TAIL geometry of p's later layouts: the first tail source and
- * the sources per tail tile (0: the defaults, 2^19 and 2^19), so that small
- * graphs run tails of many tiles. A layout built with other values is rebuilt
- * on its next run. */
-int ggpu_test_pr_tail_geometry(ggpu_pr* p, uint32_t tail_begin, uint32_t tile);
+/* Sets the HOT_TAIL geometry of p's later layouts: the first source past the
+ * split rows' heads, the sources per tail tile, the end of the band and the
+ * sources per band tile (at most 16384; band_tile 0 with band_end 1: no band).
+ * All zero: the defaults (30720, 2^19, 30720 + 30 * 16384, 16384). Small values
+ * make small graphs run bands and tails of many tiles. A layout built with
+ * other values is rebuilt on its next run. */
+int ggpu_test_pr_tail_geometry(ggpu_pr* p, uint32_t tail_begin, uint32_t tile, uint32_t band_end,
+                               uint32_t band_tile);
 
 /* The HOT_TAIL layout last built for vertex order `order` (GGPU_PR_ORDER_*):
- * stats (6 words) = split rows, tail tiles, tail arcs, tail items, split rows
- * merged by a wave, long-row chunks of the HOT items; all zero if none. */
+ * stats (8 words) = split rows, tiles, tail arcs (band included), items,
+ * split rows merged by a wave, long-row chunks of the HOT items, band tiles,
+ * band tasks; all zero if none. */
 int ggpu_test_pr_tail_info(const ggpu_pr* p, uint32_t order, uint64_t* stats);
 
 #ifdef __cplusplus

@@ -294,78 +294,108 @@ def test_hot_layout_cache_boundary():
                 assert abs(res.iters - ref_iters) <= 3
 
 
-def _tail_geometry(pr, tail_begin, tile):
-    pr.ctx.check(pr.ctx.lib.ggpu_test_pr_tail_geometry(pr._h, tail_begin, tile))
+def _tail_geometry(pr, tail_begin, tile, band_end=0, band_tile=0):
+    pr.ctx.check(pr.ctx.lib.ggpu_test_pr_tail_geometry(pr._h, tail_begin, tile, band_end, band_tile))
 
 
 def _tail_info(pr, order):
-    """(split rows, tiles, tail arcs, items, rows merged by a wave, head chunks) of the last HOT_TAIL layout."""
-    st = (ctypes.c_uint64 * 6)()
+    """(split rows, tiles, tail arcs, items, rows merged by a wave, head chunks, band tiles, band tasks)
+    of the last HOT_TAIL layout."""
+    st = (ctypes.c_uint64 * 8)()
     pr.ctx.check(pr.ctx.lib.ggpu_test_pr_tail_info(pr._h, order, st))
     return [int(x) for x in st]
 
 
-# first tail source, sources per tile, split-row in-degree (large_min)
-TAIL = [(64, 100, 1), (64, 100, 4), (16, 7, 0), (1024, 512, 2)]
+# first tail source, sources per tail tile, split-row in-degree (large_min), band end, sources per band tile
+TAIL = [(64, 100, 1, 0, 0), (64, 100, 4, 0, 0), (16, 7, 0, 0, 0), (1024, 512, 2, 0, 0),
+        (64, 300, 1, 1000, 100), (16, 7, 2, 500, 37), (0, 1000, 1, 64, 64)]
 
 
 @pytest.mark.parametrize("name", SUITE)
 def test_hot_tail_layout_matches_oracle(name):
     """HOT_TAIL with small first-tail ids and tiles (the suite graphs lie far
-    below the 2^19 default): the split rows' heads summed by the HOT sweep as
-    partials, their tails per tile by the tail kernel, each split row merged
-    from head, tile and overflow partials; both orders, several split
+    below the default geometry): the split rows' heads summed by the HOT sweep
+    as partials, their band arcs per LDS tile by the band kernel, their tails
+    per tile by the tail kernel, each split row merged from head, tile and
+    overflow partials; with and without a band, both orders, several split
     thresholds, fixed sweeps and tol mode, against the FP64 oracle with the
     acceptance rule above (L1 < 1e-4, top-1000 relative < 1e-3, |sum-1| < 1e-4)."""
     g, t, od = _in_graph(name)
     pr = G.GpuPageRank(t, od)
-    split = 0
+    split = tasks = 0
     for tol, iters in ((0.0, 20), (1e-6, 1000)):
         ref, ref_iters, _ = O.pagerank(t.row_ptr, t.col_idx, od, 0.85, iters, tol)
-        for tail_begin, tile, lmin in TAIL:
-            _tail_geometry(pr, tail_begin, tile)
+        for tail_begin, tile, lmin, band_end, band_tile in TAIL:
+            _tail_geometry(pr, tail_begin, tile, band_end, band_tile)
             for order in (G.PR_ORDER_GIVEN, G.PR_ORDER_DEGREE):
                 res = pr.run("gpu_pull_adaptive", G.pr_options(iters=iters, tol=tol, layout=G.PR_LAYOUT_HOT_TAIL,
                                                                order=order, large_min=lmin))
                 l1, rel, s = O.check_ranks(ref, pr.ranks())
                 assert l1 < 1e-4 and rel < 1e-3 and abs(s - 1) < 1e-4, \
-                    f"{name} tol {tol} tail {tail_begin}/{tile}/{lmin} order {order}: L1 {l1:.3g} rel {rel:.3g} sum {s}"
+                    f"{name} tol {tol} tail {tail_begin}/{tile}/{lmin}/{band_end}/{band_tile} order {order}: " \
+                    f"L1 {l1:.3g} rel {rel:.3g} sum {s}"
                 if tol > 0 and not res.floor_stop:
                     assert abs(res.iters - ref_iters) <= 3, (name, res.iters, ref_iters)
                 if tol == 0:
                     assert res.iters == 20 and abs(res.sum - 1.0) < 1e-4
-                split += _tail_info(pr, order)[0]
+                info = _tail_info(pr, order)
+                split += info[0]
+                tasks += info[7]
+                if not band_tile:
+                    assert info[6] == 0 and info[7] == 0
     if g.n > 64 and g.m:
         assert split > 0, "no configuration split a row"
+        assert tasks > 0, "no configuration ran the band kernel"
 
 
 def test_hot_tail_hub_segments_and_wave_merge():
     """rmat16 (65,536 vertices) in the degree order with 4096-source tiles past
-    id 4096: split hub rows whose tile segments exceed one item (overflow
-    partials), rows merged by a whole wave, rows merged by a lane; HOT_TAIL
-    equals the FP64 oracle by the acceptance rule and agrees with HOT sweep for
-    sweep (both compensated FP32, summed in different orders: within 5e-6 relative on
-    every rank)."""
+    id 4096, with and without a band of 4096- or 16384-source LDS tiles, and the
+    default geometry (three band tiles past id 30720): split hub rows whose
+    tile segments exceed one item (overflow partials), rows merged by a whole
+    wave, rows merged by a lane; HOT_TAIL equals the FP64 oracle by the
+    acceptance rule and agrees with HOT sweep for sweep (both compensated FP32,
+    summed in different orders: within 5e-6 relative on every rank)."""
     g = G.Graph.named("rmat16_c1")
     od = g.degrees()
     pr = G.GpuPageRank(g, od)
     ref, _, _ = O.pagerank(g.row_ptr, g.col_idx, od, 0.85, 20, 0.0)
     pr.run("gpu_pull_adaptive", G.pr_options(iters=20, layout=G.PR_LAYOUT_HOT, order=G.PR_ORDER_DEGREE))
     hot = pr.ranks()
-    for tail_begin, tile, lmin in ((4096, 4096, 64), (4096, 4096, 256), (1024, 1000, 1), (30720, 8192, 64)):
-        _tail_geometry(pr, tail_begin, tile)
+    geoms = [(4096, 4096, 64, 0, 0), (4096, 4096, 256, 0, 0), (1024, 1000, 1, 0, 0), (30720, 8192, 64, 0, 0),
+             (4096, 4096, 64, 4096 + 4 * 4096, 4096), (1024, 8192, 1, 1024 + 3 * 16384, 16384), (0, 0, 256, 0, 0)]
+    for tail_begin, tile, lmin, band_end, band_tile in geoms:
+        _tail_geometry(pr, tail_begin, tile, band_end, band_tile)
         for tg in (64, 1024):
             pr.run("gpu_pull_adaptive", G.pr_options(iters=20, tg=tg, layout=G.PR_LAYOUT_HOT_TAIL,
                                                      order=G.PR_ORDER_DEGREE, large_min=lmin))
             got = pr.ranks()
             assert O.ranks_ok(*O.check_ranks(ref, got)), (tail_begin, tile, lmin)
             assert np.allclose(got, hot, rtol=5e-6, atol=0), (tail_begin, tile, lmin, np.max(np.abs(got / hot - 1)))
-        R, T, tail, items, big, heads = _tail_info(pr, G.PR_ORDER_DEGREE)
+        R, T, tail, items, big, heads, band_tiles, tasks = _tail_info(pr, G.PR_ORDER_DEGREE)
         assert R > 0 and T > 1 and tail > 0 and items >= T, (R, T, tail, items)
+        if band_tile:
+            assert band_tiles == (band_end - tail_begin + band_tile - 1) // band_tile and tasks >= 1
+        if (tail_begin, tile) == (0, 0):  # the default geometry: 16384-source band tiles from 30720
+            assert band_tiles == 3 and tasks >= 1, (band_tiles, tasks)
         if (tail_begin, tile, lmin) == (4096, 4096, 64):
             assert big > 0  # hub rows with many head chunks or overflow partials
-    # the default geometry: n = 65536 < 2^19, so HOT_TAIL is HOT (no split row)
-    _tail_geometry(pr, 0, 0)
+
+
+def test_hot_tail_without_tail_sources_is_hot():
+    """rmat12 (4096 vertices) under the default geometry: no source reaches
+    id 30720, so no row is split and HOT_TAIL sums exactly as HOT does."""
+    g = G.Graph.named("rmat12")
+    pr = G.GpuPageRank(g, g.degrees())
+    pr.run("gpu_pull_adaptive", G.pr_options(iters=20, layout=G.PR_LAYOUT_HOT, order=G.PR_ORDER_DEGREE))
+    hot = pr.ranks()
     pr.run("gpu_pull_adaptive", G.pr_options(iters=20, layout=G.PR_LAYOUT_HOT_TAIL, order=G.PR_ORDER_DEGREE))
     assert _tail_info(pr, G.PR_ORDER_DEGREE)[0] == 0
     assert np.array_equal(pr.ranks(), hot)
+
+
+def test_hot_tail_geometry_hook_rejects_bad_bands():
+    g = G.Graph.named("rmat12")
+    pr = G.GpuPageRank(g, g.degrees())
+    for args in ((64, 0, 0, 0), (64, 100, 1000, 16385), (64, 100, 64, 16), (64, 100, 10, 16)):
+        assert pr.ctx.lib.ggpu_test_pr_tail_geometry(pr._h, *args) != 0, args

@@ -9,7 +9,7 @@ piece inside one tile, items of one tile in tile order within their arc and
 piece limits, every partial written at most once, every overflow partial
 written, each split at the row's first tail arc. Small first-tail ids and tile
 sizes make the suite graphs exercise many tiles, overflow pieces and items cut
-by either limit.
+by either limit, with and without a band of tiles of another size.
 """
 import ctypes as C
 
@@ -29,9 +29,9 @@ def _emulate(g, contrib, params):
     lib.ggpu_host_last_error.restype = C.c_char_p
     rp = np.ascontiguousarray(g.row_ptr, np.uint32)
     ci = np.ascontiguousarray(g.col_idx if len(g.col_idx) else np.zeros(1, np.uint32), np.uint32)
-    prm = np.ascontiguousarray(params, np.uint32)
     c = np.ascontiguousarray(contrib, np.float32)
     sums = np.zeros(max(g.n, 1), np.float64)
+    prm = np.ascontiguousarray(tuple(params) + (0, 0) * (len(params) == 5), np.uint32)
     stats = np.zeros(8, np.uint64)
     rc = f(g.n, g.m, rp.ctypes.data_as(u32p), ci.ctypes.data_as(u32p), prm.ctypes.data_as(u32p),
            c.ctypes.data_as(C.POINTER(C.c_float)), sums.ctypes.data_as(C.POINTER(C.c_double)),
@@ -52,13 +52,16 @@ def _tail_arcs(g, min_indeg, tail_begin):
     return int(((deg[rows] >= max(min_indeg, 1)) & (g.col_idx >= tail_begin)).sum())
 
 
-# min in-degree, first tail source, sources per tile, arcs per piece and item, pieces per item
-SMALL = [(4, 16, 64, 64, 8), (1, 0, 7, 64, 1), (64, 512, 256, 64, 64), (8, 100, 1000, 512, 512)]
+# min in-degree, first tail source, sources per tail tile, arcs per piece and item, pieces per item,
+# band end, sources per band tile
+SMALL = [(4, 16, 64, 64, 8, 0, 0), (1, 0, 7, 64, 1, 0, 0), (64, 512, 256, 64, 64, 0, 0), (8, 100, 1000, 512, 512, 0, 0),
+         (4, 16, 300, 64, 8, 1000, 64), (1, 0, 7, 64, 1, 50, 13), (8, 100, 1000, 512, 512, 90000, 16384)]
 
 
 @pytest.mark.parametrize("name", ["rmat12", "rmat12_directed", "star100k", "grid100x100", "complete200",
                                   "two_components", "path1000", "empty1000"])
-@pytest.mark.parametrize("params", SMALL, ids=["small", "tiny-tiles", "hub-rows", "wide"])
+@pytest.mark.parametrize("params", SMALL, ids=["small", "tiny-tiles", "hub-rows", "wide", "band", "tiny-band",
+                                                "band-to-90000"])
 def test_plan_covers_every_arc_once(name, params):
     g0 = G.Graph.named(name)
     t = g0.transpose()
@@ -66,10 +69,22 @@ def test_plan_covers_every_arc_once(name, params):
     contrib = rng.random(max(t.n, 1)).astype(np.float32)[: t.n]
     sums, stats = _emulate(t, contrib, params)
     assert np.allclose(sums, _direct(t, contrib), rtol=1e-12, atol=1e-12)
-    R, T, tail, head, pieces, items, ovf, _ = map(int, stats)
+    R, T, tail, head, pieces, items, ovf, band = map(int, stats)
     assert tail == _tail_arcs(t, params[0], params[1])
     if tail:
         assert R > 0 and T > 0 and items > 0 and pieces >= items
+    if T:
+        assert (band, T) == _tiles(t.n, params), (band, T)
+
+
+def _tiles(n, params):
+    """(band tiles, all tiles) of the geometry: the band [tail_begin, band_end) in band tiles,
+    then [band_end, n) in tail tiles (without a band, [tail_begin, n) in tail tiles)."""
+    tail_begin, tile, band_end, band_tile = params[1], params[2], params[5], params[6]
+    if not band_tile:
+        return 0, -(-(n - tail_begin) // tile)
+    band = -(-(min(n, band_end) - tail_begin) // band_tile)
+    return band, band + (-(-(n - band_end) // tile) if n > band_end else 0)
 
 
 def test_no_tail_when_every_source_is_below_the_first_tail_id():
