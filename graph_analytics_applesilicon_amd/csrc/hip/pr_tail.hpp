@@ -203,7 +203,16 @@ __device__ __forceinline__ void pr_tail_merge_body(const PrSweep& P, const PrTai
         if (h.y + (o1 - o0) > kMergeLanePartials) continue;  // in `big`
         float H = 0.f, L = 0.f;
         for (uint32_t q = 0; q < h.y; ++q) tail_add(H, L, P.stream.lpart[h.x + q]);
-        for (uint32_t t = 0; t < Z.T; ++t) tail_add(H, L, Z.part[(size_t)t * Z.R + j]);
+        // tile partials eight at a time: their loads (clamped to the last tile, unconditional) are
+        // issued before the adds, which keep tile order
+        for (uint32_t t0 = 0; t0 < Z.T; t0 += 8) {
+            float2 x[8];
+#pragma unroll
+            for (uint32_t i = 0; i < 8; ++i) x[i] = Z.part[(size_t)min(t0 + i, Z.T - 1) * Z.R + j];
+#pragma unroll
+            for (uint32_t i = 0; i < 8; ++i)
+                if (t0 + i < Z.T) tail_add(H, L, x[i]);
+        }
         for (uint32_t o = o0; o < o1; ++o) tail_add(H, L, Z.part[(size_t)Z.T * Z.R + o]);
         if (Z.srow[j] < P.n) pr_store(P, Z.srow[j], add + d * (H + L), part, inv_n);
     }
