@@ -65,18 +65,18 @@ __device__ __forceinline__ void pr_tail_body(const PrSweep& P, const PrTailDev& 
     const uint32_t x0 = Z.px[item.x], cnt = Z.px[item.y] - x0;
     float* st = s_stage[w];
     for (uint32_t base = 0; base < cnt; base += kGatherBatch * kWave) {
+        // unconditional loads (lanes past the item re-read its last arc; cnt >= 1), so
+        // the compiler issues the pass's gathers together (masked: 839 against 826 us
+        // per rmat23 sweep)
         uint32_t u[kGatherBatch];
 #pragma unroll
         for (uint32_t j = 0; j < kGatherBatch; ++j) {
             const uint32_t i = base + j * kWave + lane;
-            u[j] = i < cnt ? __builtin_nontemporal_load(&Z.tci[x0 + i]) : 0u;
+            u[j] = __builtin_nontemporal_load(&Z.tci[x0 + min(i, cnt - 1)]);
         }
         float v[kGatherBatch];
 #pragma unroll
-        for (uint32_t j = 0; j < kGatherBatch; ++j) {
-            const uint32_t i = base + j * kWave + lane;
-            v[j] = i < cnt ? P.contrib_in[u[j]] : 0.0f;
-        }
+        for (uint32_t j = 0; j < kGatherBatch; ++j) v[j] = P.contrib_in[u[j]];
 #pragma unroll
         for (uint32_t j = 0; j < kGatherBatch; ++j) {
             const uint32_t i = base + j * kWave + lane;
