@@ -126,6 +126,22 @@ __device__ __forceinline__ void pr_store(const PrSweep& P, uint32_t v, float val
     }
 }
 
+// pr_store with the vertex's out-degree and previous rank already loaded (0 when
+// the sweep keeps no previous ranks).
+__device__ __forceinline__ void pr_store_pre(const PrSweep& P, uint32_t v, float val, uint32_t od, float old,
+                                             PrPart& part, float inv_n) {
+    P.r_new[v] = val;
+    P.contrib_out[v] = od ? val / (float)od : 0.0f;
+    if (!od) part.dang += (double)val;
+    part.sum += (double)val;
+    if (P.r_old) {
+        const float dlt = fabsf(val - old);
+        part.delta += (double)dlt;
+        const double rel = (double)dlt / (double)fmaxf(old, inv_n);
+        part.maxrel = rel > part.maxrel ? rel : part.maxrel;
+    }
+}
+
 __device__ __forceinline__ float in_sum_lane(const PrSweep& P, uint32_t v) {
     Kahan k;
     const uint32_t b = P.rp[v], e = P.rp[v + 1];
@@ -220,24 +236,34 @@ __device__ __forceinline__ void pr_stream_body(const PrSweep& P, const PrStreamL
             continue;
         }
         const uint2 si = Lyt.sitems[it - Lyt.n_litems];
-        const uint32_t r0 = si.x, r1 = si.y;
+        const uint32_t r0 = si.x, r1 = si.y;  // r1 > r0
+        const uint32_t rows = r1 - r0;
+        uint32_t G = T / rows;
+        G = G >= 64 ? 64 : (G >= 32 ? 32 : (G >= 16 ? 16 : (G >= 8 ? 8 : (G >= 4 ? 4 : (G >= 2 ? 2 : 1)))));
+        const uint32_t per_round = T / G, rr = threadIdx.x / G, gl = threadIdx.x % G;
+        // row bounds, out-degree and previous rank one round ahead, as in pr_hot_body
+        uint32_t rc = min(r0 + rr, r1 - 1);
+        uint32_t ra = P.rp[rc], rz = P.rp[rc + 1], od = P.outdeg[rc];
+        float old = P.r_old ? P.r_old[rc] : 0.0f;
         const uint32_t x0 = P.rp[r0], x1 = P.rp[r1];
         gather_batched(P, x0, x1 - x0, [&](uint32_t i, float x) { s_val[i] = x; });
         __syncthreads();
-        const uint32_t rows = r1 - r0;
-        uint32_t G = T / (rows ? rows : 1);
-        G = G >= 64 ? 64 : (G >= 32 ? 32 : (G >= 16 ? 16 : (G >= 8 ? 8 : (G >= 4 ? 4 : (G >= 2 ? 2 : 1)))));
-        const uint32_t per_round = T / G, rr = threadIdx.x / G, gl = threadIdx.x % G;
         for (uint32_t rb = 0; rb < rows; rb += per_round) {
             const uint32_t r = r0 + rb + rr;
+            const bool ok = r < r1;
+            const uint32_t ca = ra, cz = rz, cod = od;
+            const float cold = old;
+            rc = min(r + per_round, r1 - 1);
+            ra = P.rp[rc];
+            rz = P.rp[rc + 1];
+            od = P.outdeg[rc];
+            old = P.r_old ? P.r_old[rc] : 0.0f;
             Kahan k;
-            if (r < r1) {
-                const uint32_t a = P.rp[r] - x0, b = P.rp[r + 1] - x0;
-                for (uint32_t x = a + gl; x < b; x += G) k.add(s_val[x]);
-            }
+            if (ok)
+                for (uint32_t x = ca - x0 + gl; x < cz - x0; x += G) k.add(s_val[x]);
             float hi = k.s, lo = -k.c;
             comp_reduce(hi, lo, G);
-            if (r < r1 && gl == 0) pr_store(P, r, add + d * (hi + lo), part, inv_n);
+            if (ok && gl == 0) pr_store_pre(P, r, add + d * (hi + lo), cod, cold, part, inv_n);
         }
         __syncthreads();
     }
@@ -319,24 +345,37 @@ __device__ __forceinline__ void pr_hot_body(const PrSweep& P, float add, float d
             continue;
         }
         const uint2 si = Lyt.sitems[it - Lyt.n_litems];
-        const uint32_t r0 = si.x, r1 = si.y;
-        const uint32_t x0 = P.rp[r0], x1 = P.rp[r1];
-        gather_wave(P, s_hot, hot, x0, x1 - x0, [&](uint32_t i, float x) { st[i] = x; });
-        wave_lds_sync();
+        const uint32_t r0 = si.x, r1 = si.y;  // r1 > r0
         const uint32_t rows = r1 - r0;
         uint32_t G = rows >= kWave ? 1u : kWave / rows;
         G = G >= 64 ? 64 : (G >= 32 ? 32 : (G >= 16 ? 16 : (G >= 8 ? 8 : (G >= 4 ? 4 : (G >= 2 ? 2 : 1)))));
         const uint32_t per_round = kWave / G, rr = lane / G, gl = lane % G;
+        // a row's bounds, out-degree and previous rank are loaded one round ahead (the
+        // first round's beside the gathers; rows clamped to the block's last, loads
+        // unconditional), so a round of row sums waits on no load of its own: 981 ->
+        // 965 us per HOT sweep and 806 -> 791 us per HOT_TAIL sweep on rmat23
+        uint32_t rc = min(r0 + rr, r1 - 1);
+        uint32_t ra = P.rp[rc], rz = P.rp[rc + 1], od = P.outdeg[rc];
+        float old = P.r_old ? P.r_old[rc] : 0.0f;
+        const uint32_t x0 = P.rp[r0], x1 = P.rp[r1];
+        gather_wave(P, s_hot, hot, x0, x1 - x0, [&](uint32_t i, float x) { st[i] = x; });
+        wave_lds_sync();
         for (uint32_t rb = 0; rb < rows; rb += per_round) {
             const uint32_t r = r0 + rb + rr;
+            const bool ok = r < r1;
+            const uint32_t ca = ra, cz = rz, cod = od;
+            const float cold = old;
+            rc = min(r + per_round, r1 - 1);
+            ra = P.rp[rc];
+            rz = P.rp[rc + 1];
+            od = P.outdeg[rc];
+            old = P.r_old ? P.r_old[rc] : 0.0f;
             Kahan k;
-            if (r < r1) {
-                const uint32_t a = P.rp[r] - x0, b = P.rp[r + 1] - x0;
-                for (uint32_t x = a + gl; x < b; x += G) k.add(st[x]);
-            }
+            if (ok)
+                for (uint32_t x = ca - x0 + gl; x < cz - x0; x += G) k.add(st[x]);
             float hi = k.s, lo = -k.c;
             comp_reduce(hi, lo, G);
-            if (r < r1 && gl == 0) pr_store(P, r, add + d * (hi + lo), part, inv_n);
+            if (ok && gl == 0) pr_store_pre(P, r, add + d * (hi + lo), cod, cold, part, inv_n);
         }
         wave_lds_sync();
     }
