@@ -223,11 +223,11 @@ __device__ __forceinline__ void pr_stream_body(const PrSweep& P, const PrStreamL
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         if (it < Lyt.n_litems) {
             const uint4 li = Lyt.litems[it];
+            const uint2 lr = Lyt.lrows[li.w];  // loaded beside the gathers, not after the sum
             Kahan k;
             gather_batched(P, li.y, li.z - li.y, [&](uint32_t, float x) { k.add(x); });
             float hi = k.s, lo = -k.c;
             block_comp_reduce(hi, lo, s_hi, s_lo);
-            const uint2 lr = Lyt.lrows[li.w];
             if (threadIdx.x == 0) {
                 if (lr.y == 1) pr_store(P, li.x, add + d * (hi + lo), part, inv_n);
                 else Lyt.lpart[it] = make_float2(hi, lo);  // combined by pr_longfix_body
@@ -334,12 +334,13 @@ __device__ __forceinline__ void pr_hot_body(const PrSweep& P, float add, float d
     for (uint32_t it = blockIdx.x * kHotWaves + w; it < n_items; it += gridDim.x * kHotWaves) {
         if (it < Lyt.n_litems) {
             const uint4 li = Lyt.litems[it];
+            const uint32_t chunks = Lyt.lrows[li.w].y;  // loaded beside the gathers, not after the sum
             Kahan k;
             gather_wave(P, s_hot, hot, li.y, li.z - li.y, [&](uint32_t, float x) { k.add(x); });
             float hi = k.s, lo = -k.c;
             comp_reduce(hi, lo, kWave);
             if (lane == 0) {
-                if (Lyt.lrows[li.w].y == 1) pr_store(P, li.x, add + d * (hi + lo), part, inv_n);
+                if (chunks == 1) pr_store(P, li.x, add + d * (hi + lo), part, inv_n);
                 else Lyt.lpart[it] = make_float2(hi, lo);  // combined by pr_longfix_body
             }
             continue;
