@@ -37,10 +37,12 @@ int ggpu_test_msbfs_route(ggpu_bfs* b, uint64_t min_edges, uint32_t* routed_last
 
 /* Sets the HOT_TAIL geometry of p's later layouts: the first source past the
  * split rows' heads, the sources per tail tile, the end of the band and the
- * sources per band tile (at most 16384; band_tile 0 with band_end 1: no band).
- * All zero: the defaults (30720, 2^19, 30720 + 30 * 16384, 16384). Small values
- * make small graphs run bands and tails of many tiles. A layout built with
- * other values is rebuilt on its next run. */
+ * sources per band tile (at most 16384; 0: no band). tail_begin and tile both
+ * 0: the defaults (30720, 2^19, 30720 + 30 * 16384, 16384), whatever the band
+ * arguments. Small values make small graphs run bands and tails of many tiles.
+ * GGPU_EINVAL for tile 0 with tail_begin > 0, band_tile > 16384, or a band that
+ * does not end after tail_begin. A layout built with other values is rebuilt on
+ * its next run. */
 int ggpu_test_pr_tail_geometry(ggpu_pr* p, uint32_t tail_begin, uint32_t tile, uint32_t band_end,
                                uint32_t band_tile);
 
