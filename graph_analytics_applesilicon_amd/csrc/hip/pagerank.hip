@@ -46,6 +46,10 @@ constexpr uint32_t kTailBegin = kHotCap;
 constexpr uint32_t kBandTiles = 30;
 constexpr uint32_t kTailTile = 1u << 19;
 constexpr uint32_t kTailMergeBlocks = 16384;       // block cap of the tail merge (grid-stride beyond)
+// arcs per chunk of a split row's head (LDS gathers only; a wave sums a chunk in
+// passes of kWaveNnz): 512 / 1024 / 2048 / 4096 -> 772 / 759 / 762 / 759 us per
+// HOT_TAIL sweep on rmat23 (fewer chunks, partials and merge reads)
+constexpr uint32_t kHeadChunk = 1024;
 
 // ------------------------------------------------------------------ kernels
 __global__ void k_pr_sweep(PrSweep P) { pr_sweep_body(P); }
@@ -543,8 +547,9 @@ struct PrEngine {
         };
         auto chunks = [&](uint32_t v, uint32_t b, uint32_t e, uint32_t flag) {
             const uint32_t first = (uint32_t)litems.size();
-            for (uint32_t x = b; x < e; x += kWaveNnz)
-                litems.push_back(make_uint4(v, x, std::min(x + kWaveNnz, e), (uint32_t)lrows.size()));
+            const uint32_t cl = flag ? kHeadChunk : kWaveNnz;
+            for (uint32_t x = b; x < e; x += cl)
+                litems.push_back(make_uint4(v, x, std::min(x + cl, e), (uint32_t)lrows.size()));
             const uint32_t cnt = (uint32_t)litems.size() - first;
             lrows.push_back(make_uint2(first, cnt | flag));
             return cnt;
