@@ -157,7 +157,8 @@ __global__ void k_pr_finish(float* r_new, const float* r_old, float* contrib, co
 // Row-block items of the stream layout over rows [r_begin, r_end).
 struct StreamLayout {
     uint32_t r_begin = 0, r_end = 0;
-    DBuf<uint2> sitems, lrows;
+    DBuf<uint4> sitems;
+    DBuf<uint2> lrows;
     DBuf<uint4> litems;
     DBuf<float2> lpart;
     uint32_t ns = 0, nl = 0, nlr = 0;
@@ -416,11 +417,11 @@ struct PrEngine {
         auto L = std::make_unique<StreamLayout>();
         L->r_begin = ra;
         L->r_end = rb;
-        std::vector<uint2> sitems, lrows;
-        std::vector<uint4> litems;
+        std::vector<uint4> sitems, litems;
+        std::vector<uint2> lrows;
         uint32_t cur0 = ra, rows = 0, nnz = 0;
         auto close = [&](uint32_t end) {
-            if (rows) sitems.push_back(make_uint2(cur0, end));
+            if (rows) sitems.push_back(make_uint4(cur0, end, S.in.rp[cur0], S.in.rp[end]));
             rows = 0;
             nnz = 0;
         };
@@ -448,7 +449,7 @@ struct PrEngine {
         L->lrows.alloc(L->nlr + 1);
         L->lpart.alloc(L->nl + 1);
         for (const uint2& lr : lrows) L->multi |= lr.y > 1;
-        if (L->ns) GGPU_HIP_TRY(hipMemcpy(L->sitems.p, sitems.data(), L->ns * sizeof(uint2), hipMemcpyHostToDevice));
+        if (L->ns) GGPU_HIP_TRY(hipMemcpy(L->sitems.p, sitems.data(), L->ns * sizeof(uint4), hipMemcpyHostToDevice));
         if (L->nl) GGPU_HIP_TRY(hipMemcpy(L->litems.p, litems.data(), L->nl * sizeof(uint4), hipMemcpyHostToDevice));
         if (L->nlr) GGPU_HIP_TRY(hipMemcpy(L->lrows.p, lrows.data(), L->nlr * sizeof(uint2), hipMemcpyHostToDevice));
         return L;
@@ -535,13 +536,13 @@ struct PrEngine {
         T->T = plan.T;
         T->n_items = (uint32_t)plan.items.size();
         T->tail_arcs = plan.tci.size();
-        std::vector<uint2> sitems, lrows;
-        std::vector<uint4> litems;
+        std::vector<uint4> sitems, litems;
+        std::vector<uint2> lrows;
         std::vector<uint32_t> slr(plan.R, kNoHead);
         uint32_t cur0 = 0, rows = 0, nnz = 0, j = 0;
         bool multi = false;
         auto close = [&](uint32_t end) {
-            if (rows) sitems.push_back(make_uint2(cur0, end));
+            if (rows) sitems.push_back(make_uint4(cur0, end, S.in.rp[cur0], S.in.rp[end]));
             rows = 0;
             nnz = 0;
         };

@@ -40,7 +40,7 @@ struct PrSection {
 constexpr uint32_t kLrowSplit = 0x80000000u;
 
 struct PrStreamLayout {
-    const uint2* sitems;  // {first row, end row}
+    const uint4* sitems;  // {first row, end row, first edge, end edge}
     uint32_t n_sitems;
     const uint4* litems;  // {row, first edge, end edge, long-row index}
     uint32_t n_litems;
@@ -235,7 +235,7 @@ __device__ __forceinline__ void pr_stream_body(const PrSweep& P, const PrStreamL
             __syncthreads();
             continue;
         }
-        const uint2 si = Lyt.sitems[it - Lyt.n_litems];
+        const uint4 si = Lyt.sitems[it - Lyt.n_litems];
         const uint32_t r0 = si.x, r1 = si.y;  // r1 > r0
         const uint32_t rows = r1 - r0;
         uint32_t G = T / rows;
@@ -245,7 +245,7 @@ __device__ __forceinline__ void pr_stream_body(const PrSweep& P, const PrStreamL
         uint32_t rc = min(r0 + rr, r1 - 1);
         uint32_t ra = P.rp[rc], rz = P.rp[rc + 1], od = P.outdeg[rc];
         float old = P.r_old ? P.r_old[rc] : 0.0f;
-        const uint32_t x0 = P.rp[r0], x1 = P.rp[r1];
+        const uint32_t x0 = si.z, x1 = si.w;  // rp[r0], rp[r1], in the item: no load round
         gather_batched(P, x0, x1 - x0, [&](uint32_t i, float x) { s_val[i] = x; });
         __syncthreads();
         for (uint32_t rb = 0; rb < rows; rb += per_round) {
@@ -345,7 +345,7 @@ __device__ __forceinline__ void pr_hot_body(const PrSweep& P, float add, float d
             }
             continue;
         }
-        const uint2 si = Lyt.sitems[it - Lyt.n_litems];
+        const uint4 si = Lyt.sitems[it - Lyt.n_litems];
         const uint32_t r0 = si.x, r1 = si.y;  // r1 > r0
         const uint32_t rows = r1 - r0;
         uint32_t G = rows >= kWave ? 1u : kWave / rows;
@@ -358,7 +358,7 @@ __device__ __forceinline__ void pr_hot_body(const PrSweep& P, float add, float d
         uint32_t rc = min(r0 + rr, r1 - 1);
         uint32_t ra = P.rp[rc], rz = P.rp[rc + 1], od = P.outdeg[rc];
         float old = P.r_old ? P.r_old[rc] : 0.0f;
-        const uint32_t x0 = P.rp[r0], x1 = P.rp[r1];
+        const uint32_t x0 = si.z, x1 = si.w;  // rp[r0], rp[r1], in the item: no load round
         gather_wave(P, s_hot, hot, x0, x1 - x0, [&](uint32_t i, float x) { st[i] = x; });
         wave_lds_sync();
         for (uint32_t rb = 0; rb < rows; rb += per_round) {
