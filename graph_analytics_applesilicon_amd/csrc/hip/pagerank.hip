@@ -161,6 +161,7 @@ struct StreamLayout {
     DBuf<uint2> lrows;
     DBuf<uint4> litems;
     DBuf<float2> lpart;
+    DBuf<uint16_t> h16;  // HOT_TAIL: the split rows' head arc ids (items flagged kLiHead16)
     uint32_t ns = 0, nl = 0, nlr = 0;
     bool multi = false;  // some long row has more than one chunk
 };
@@ -546,11 +547,25 @@ struct PrEngine {
             rows = 0;
             nnz = 0;
         };
+        // heads with every source below the LDS copy's end keep their arc ids as u16
+        static_assert(kHotCap <= 65536, "head arc ids must fit in 16 bits");
+        const bool head16 = tb <= std::min<uint32_t>(n, kHotCap);
+        std::vector<uint16_t> h16;
         auto chunks = [&](uint32_t v, uint32_t b, uint32_t e, uint32_t flag) {
             const uint32_t first = (uint32_t)litems.size();
             const uint32_t cl = flag ? kHeadChunk : kWaveNnz;
-            for (uint32_t x = b; x < e; x += cl)
-                litems.push_back(make_uint4(v, x, std::min(x + cl, e), (uint32_t)lrows.size()));
+            if (lrows.size() >= kLiHead16) throw ArgError("too many long rows for the HOT_TAIL layout");
+            for (uint32_t x = b; x < e; x += cl) {
+                const uint32_t xe = std::min(x + cl, e);
+                if (flag && head16) {
+                    if (h16.size() & 1) h16.push_back(0);  // chunks start at even offsets (32-bit pair loads)
+                    const uint32_t o = (uint32_t)h16.size();
+                    for (uint32_t y = x; y < xe; ++y) h16.push_back((uint16_t)S.in.ci[y]);
+                    litems.push_back(make_uint4(v, o, o + (xe - x), (uint32_t)lrows.size() | kLiHead16));
+                } else {
+                    litems.push_back(make_uint4(v, x, xe, (uint32_t)lrows.size()));
+                }
+            }
             const uint32_t cnt = (uint32_t)litems.size() - first;
             lrows.push_back(make_uint2(first, cnt | flag));
             return cnt;
@@ -587,6 +602,8 @@ struct PrEngine {
         upload_vec(L.sitems, sitems);
         upload_vec(L.litems, litems);
         upload_vec(L.lrows, lrows);
+        if (h16.size() & 1) h16.push_back(0);  // a chunk's last pair load stays in the array
+        upload_vec(L.h16, h16);
         L.lpart.alloc(L.nl + 1);
         // split rows with many head or overflow partials are merged by a wave
         std::vector<uint32_t> big;
@@ -728,6 +745,7 @@ struct PrEngine {
         Y.n_sitems = L.ns;
         Y.litems = L.litems.p;
         Y.n_litems = L.nl;
+        Y.hci16 = L.h16.p;
         Y.lrows = L.lrows.p;
         Y.lpart = L.lpart.p;
         Y.n_lrows = L.nlr;

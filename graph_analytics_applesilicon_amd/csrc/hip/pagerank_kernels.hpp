@@ -38,11 +38,17 @@ struct PrSection {
 // its chunk partials are always stored, and the tail merge (pr_tail.hpp), not
 // pr_longfix_body, combines them with the row's tail partials.
 constexpr uint32_t kLrowSplit = 0x80000000u;
+// Set in a long item's fourth word (the long-row index) when the item is a split
+// row's head chunk whose arc ids are stored as u16 in hci16[y, z), y even (every source
+// of a head lies below the LDS copy's end, which is below 2^16); only the HOT
+// sweep runs such items.
+constexpr uint32_t kLiHead16 = 0x80000000u;
 
 struct PrStreamLayout {
     const uint4* sitems;  // {first row, end row, first edge, end edge}
     uint32_t n_sitems;
     const uint4* litems;  // {row, first edge, end edge, long-row index}
+    const uint16_t* hci16;  // the head chunks' arc ids (items flagged kLiHead16)
     uint32_t n_litems;
     const uint2* lrows;   // {first item, item count (| kLrowSplit)} per long row
     float2* lpart;        // per long item (hi, lo)
@@ -321,6 +327,32 @@ __device__ __forceinline__ void gather_wave(const PrSweep& P, const float* s_hot
     }
 }
 
+// A split row's head chunk: u16 arc ids hci16[x0, x0 + cnt) (x0 even, the
+// array padded to an even length), two per 32-bit load, every source read from
+// the LDS copy (the clamp only keeps a read in bounds).
+__device__ __forceinline__ void gather_head16(const uint16_t* hci16, const float* s_hot, uint32_t hot, uint32_t x0,
+                                              uint32_t cnt, Kahan& k) {
+    const uint32_t lane = lane_id();
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(hci16 + x0);
+    const uint32_t pairs = (cnt + 1) / 2;
+    for (uint32_t base = 0; base < pairs; base += kGatherBatch * kWave) {
+        uint32_t u[kGatherBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            const uint32_t i = base + j * kWave + lane;
+            u[j] = i < pairs ? __builtin_nontemporal_load(&w[i]) : 0u;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kGatherBatch; ++j) {
+            const uint32_t i = base + j * kWave + lane;
+            if (i < pairs) {
+                k.add(s_hot[min(u[j] & 0xFFFFu, hot - 1)]);
+                if (2 * i + 1 < cnt) k.add(s_hot[min(u[j] >> 16, hot - 1)]);
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ void pr_hot_body(const PrSweep& P, float add, float d, float inv_n, PrPart& part) {
     __shared__ float s_hot[kHotCap];
     __shared__ float s_stage[kHotWaves][kWaveNnz];
@@ -334,9 +366,10 @@ __device__ __forceinline__ void pr_hot_body(const PrSweep& P, float add, float d
     for (uint32_t it = blockIdx.x * kHotWaves + w; it < n_items; it += gridDim.x * kHotWaves) {
         if (it < Lyt.n_litems) {
             const uint4 li = Lyt.litems[it];
-            const uint32_t chunks = Lyt.lrows[li.w].y;  // loaded beside the gathers, not after the sum
+            const uint32_t chunks = Lyt.lrows[li.w & ~kLiHead16].y;  // loaded beside the gathers, not after the sum
             Kahan k;
-            gather_wave(P, s_hot, hot, li.y, li.z - li.y, [&](uint32_t, float x) { k.add(x); });
+            if (li.w & kLiHead16) gather_head16(Lyt.hci16, s_hot, hot, li.y, li.z - li.y, k);
+            else gather_wave(P, s_hot, hot, li.y, li.z - li.y, [&](uint32_t, float x) { k.add(x); });
             float hi = k.s, lo = -k.c;
             comp_reduce(hi, lo, kWave);
             if (lane == 0) {
